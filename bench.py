@@ -1,0 +1,215 @@
+"""BP5 matrix-free operator benchmark on one MI355X.
+
+Workload (one "step"): one operator apply (vmult, deal.II semantics: dst = 0,
+gather/apply/scatter, constrained rows copied) for each of p = 4, 6, 8 of the
+BP5 plain method: scalar Laplacian, Q_p with GLL collocation (q = p+1), unit
+cube deformed by x += 0.05 sin(2 pi x) sin(2 pi y) sin(2 pi z), n^3 cells with
+n = 79 / 53 / 40 (about 32M DoFs each), FP64, input uniform[-1,1] from
+SplitMix64 seed 1.  Value = aggregate GDoF/s = sum_p N_p / sum_p t_p.
+Also reported per degree: the apply kernel's roofline (algorithmic bytes /
+kernel time, from HIP events on the launch stream) and Jacobi-CG
+time-to-1e-8 relative residual.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+FP64_PEAK = 78.6e12
+CELLS = {2: 158, 4: 79, 6: 53, 8: 40}
+
+
+def algorithmic_bytes(p, N, E):
+    """src read + dst write (8 B each per DoF), uint32 map (4 (p+1)^3 per cell),
+    stored G (6 doubles per quadrature point, q = p+1)."""
+    q = p + 1
+    return 16 * N + 4 * (p + 1) ** 3 * E + 48 * q ** 3 * E
+
+
+def flops_per_cell(p):
+    q = p + 1
+    return 12 * q ** 4 + 15 * q ** 3
+
+
+def bench_degree(p, steps, warmup, do_cg):
+    import torch
+
+    from benchmarks_amd.laplace import (DiagonalMatrix, LaplaceOperator, ReductionControl,
+                                        SolverCG, bk5_apply, random_vector)
+    n = CELLS[p]
+    op = LaplaceOperator(p, p + 1, 1, n_cells=n, deform=0.05)
+    N, E = op.m(), op.n_cells()
+    x = random_vector(N, 1)
+    y = op.initialize_dof_vector()
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        op.vmult(y, x)
+    torch.cuda.synchronize()
+    # whole vmult (what a solver sees)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        op.vmult(y, x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_vmult = e0.elapsed_time(e1) * 1e-3 / steps
+    # the apply kernel alone (kernel-level entry point, same stream)
+    D = op.device_array("D", torch.float64, (p + 1) ** 2)
+    G = op.device_array("G", torch.float64, E * 6 * (p + 1) ** 3)
+    mp = op.device_array("map", torch.int32, E * (p + 1) ** 3)
+    scratch = torch.zeros(N, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        bk5_apply(p, D, G, x, scratch, mp, E)
+    torch.cuda.synchronize()
+    k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    k0.record(stream)
+    for _ in range(steps):
+        bk5_apply(p, D, G, x, scratch, mp, E)
+    k1.record(stream)
+    torch.cuda.synchronize()
+    t_kernel = k0.elapsed_time(k1) * 1e-3 / steps
+    del D, G, mp, scratch
+    res = {"p": p, "n_cells_1d": n, "n_dofs": N, "n_cells": E, "ms_vmult": t_vmult * 1e3,
+           "gdofs_vmult": N / t_vmult * 1e-9, "ms_kernel": t_kernel * 1e3,
+           "bytes_alg": algorithmic_bytes(p, N, E),
+           "kernel_GBs": algorithmic_bytes(p, N, E) / t_kernel * 1e-9,
+           "kernel_TFLOPs": flops_per_cell(p) * E / t_kernel * 1e-12}
+    if do_cg:
+        b = random_vector(N, 1)
+        op.zero_constrained(b)
+        inv = op.compute_inverse_diagonal()
+        sol = op.initialize_dof_vector()
+        ctl = ReductionControl(100000, 1e-300, 1e-8)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        SolverCG(ctl).solve(op, sol, b, DiagonalMatrix(inv))
+        torch.cuda.synchronize()
+        res.update({"cg_jacobi_its": ctl.last_step(), "cg_jacobi_s": time.perf_counter() - t0,
+                    "cg_final_rel_res": ctl.last_value() / ctl.initial_value()})
+    del op, x, y
+    torch.cuda.empty_cache()
+    return res
+
+
+def _cpu_worker(args):
+    """One host process: the reference kernel on its own cell slice for about
+    `seconds`; returns (cells processed, elapsed).  A process per slice
+    because the reference kernel returns before its delete[] (it leaks its
+    three q^3 scratch arrays per cell); the memory goes with the process."""
+    p, cells, seconds, use_ref = args
+    import numpy as np
+
+    from oracle import oracle as O
+    nq = p + 1
+    _, D, _, _ = O.basis_tables(p, nq, O.GLL)
+    rng = np.random.default_rng(p)
+    G = rng.standard_normal(cells * 6 * nq ** 3)
+    u = rng.standard_normal(cells * nq ** 3)
+    out = np.zeros(cells * nq ** 3)
+    fn = O.ref_lib(fast=True).ref_bk5_serial if use_ref else None
+    done, t0 = 0, time.perf_counter()
+    while True:
+        if fn is not None:
+            fn(nq, cells, D, G, u, out)
+        else:
+            out[:] = O.bk5_evec(nq, cells, D, G, u)
+        done += cells
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            return done, el
+
+
+def cpu_baseline(degrees, budget_s=6.0):
+    """The reference's own serial BK5 kernel
+    (sum_factorization/include/kernels/BK5/serial_kernels.hpp:9-87, built by
+    oracle/Makefile into oracle/_ref/) on T host processes, each on its own
+    slice of cells (E-vector form: the reference CPU path has no gather or
+    scatter); the per-cell time is extrapolated to each degree's full apply."""
+    import multiprocessing as mp
+
+    from oracle import oracle as O
+    use_ref = True
+    try:
+        O.ref_lib(fast=True)
+    except (FileNotFoundError, OSError):
+        use_ref = False
+    procs = max(1, min(16, os.cpu_count() or 1))
+    per_deg = budget_s / len(degrees)
+    t_tot, n_tot, samples = 0.0, 0, []
+    ctx = mp.get_context("fork")
+    for p in degrees:
+        nq = p + 1
+        n = CELLS[p]
+        E, N = n ** 3, (n * p + 1) ** 3
+        cells = max(32, int(512 * (5 / nq) ** 3))
+        with ctx.Pool(procs) as pool:
+            res = pool.map(_cpu_worker, [(p, cells, per_deg, use_ref)] * procs)
+        done = sum(r[0] for r in res)
+        wall = max(r[1] for r in res)
+        t_tot += wall / done * E
+        n_tot += N
+        samples.append(f"p={p}: {done} cells in {wall:.1f}s")
+    return {"value": n_tot / t_tot * 1e-9, "unit": "GDoF/s", "cores": procs,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"{'reference' if use_ref else 'oracle'} BK5 serial kernel (E-vector, no "
+                      f"gather/scatter), {procs} processes on disjoint cell slices, per-cell time "
+                      "extrapolated to the full p=4,6,8 applies; " + "; ".join(samples)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--degrees", default="4,6,8")
+    ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the multi-GPU path (slab decomposition with halo "
+              "exchange) is designed but not built; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    degrees = [int(d) for d in args.degrees.split(",")]
+    # host baseline first, before anything initialises the GPU in this process
+    cpu = None if args.no_cpu_baseline else cpu_baseline(degrees)
+    import torch
+    torch.cuda.synchronize()
+    per = [bench_degree(p, args.steps, args.warmup, not args.no_cg) for p in degrees]
+    t_sum = sum(r["ms_vmult"] for r in per) * 1e-3
+    n_sum = sum(r["n_dofs"] for r in per)
+    dom = max(per, key=lambda r: r["ms_kernel"])
+    achieved = dom["bytes_alg"] / (dom["ms_kernel"] * 1e-3)
+    out = {
+        "metric": "BP5 matvec GDoF/s at p=4-8 (aggregate over p=4,6,8); CG time-to-1e-8 per degree",
+        "value": n_sum / t_sum * 1e-9,
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_sum * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (deformed unit-cube mesh, SplitMix64 seed-1 input)",
+        "config": {"workload": "BP5 GLL q=p+1, deformed unit cube, p=4/6/8 on 79^3/53^3/40^3 "
+                               "cells (~32M DoFs each), stored geometry, FP64 vmult",
+                   "degrees": degrees, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved * 1e-9, "peak": HBM_PEAK * 1e-9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": f"bk5 apply p={dom['p']}", "ms_per_launch": dom["ms_kernel"]},
+        "per_degree": per,
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

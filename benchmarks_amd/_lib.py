@@ -1,0 +1,116 @@
+"""ctypes binding of the C-ABI in include/mfx.h (benchmarks_amd/libmfx.so).
+
+The library is the product: there is no fallback.  If libmfx.so is missing
+every entry point raises, naming the build step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmfx.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "mfx.h")
+
+MFX_OK = 0
+STATUS_NAMES = {0: "MFX_OK", 1: "MFX_ERR_INVALID_ARG", 2: "MFX_ERR_OOM", 3: "MFX_ERR_HIP",
+                4: "MFX_ERR_NOT_CONVERGED", 5: "MFX_ERR_UNSUPPORTED", 6: "MFX_ERR_BREAKDOWN"}
+INVALID_DOF = 0xFFFFFFFF
+QUAD_GAUSS, QUAD_GLL = 0, 1
+GEOM_STORED, GEOM_ON_THE_FLY = 0, 1
+
+
+class MfxError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class BoxDesc(C.Structure):
+    _fields_ = [("degree", C.c_int), ("n_q_points_1d", C.c_int), ("quadrature", C.c_int),
+                ("geometry", C.c_int), ("n_cells", C.c_uint32 * 3), ("lo", C.c_double * 3),
+                ("hi", C.c_double * 3), ("deform", C.c_double), ("with_fp32", C.c_int)]
+
+
+class OpDesc(C.Structure):
+    _fields_ = [("degree", C.c_int), ("n_q_points_1d", C.c_int), ("quadrature", C.c_int),
+                ("geometry", C.c_int), ("n_cells", C.c_uint32), ("n_dofs", C.c_uint64),
+                ("map", C.c_void_p), ("cell_vertices", C.c_void_p),
+                ("constrained", C.c_void_p), ("n_constrained", C.c_uint64),
+                ("with_fp32", C.c_int)]
+
+
+class OpInfo(C.Structure):
+    _fields_ = [("degree", C.c_int), ("n_q_points_1d", C.c_int), ("quadrature", C.c_int),
+                ("geometry", C.c_int), ("n_cells", C.c_uint32), ("n_dofs", C.c_uint64),
+                ("n_constrained", C.c_uint64), ("map", C.c_void_p), ("G", C.c_void_p),
+                ("B", C.c_void_p), ("D", C.c_void_p), ("cell_vertices", C.c_void_p),
+                ("constrained", C.c_void_p), ("xq", C.c_void_p), ("wq", C.c_void_p)]
+
+
+class CgResult(C.Structure):
+    _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("initial_value", C.c_double),
+                ("last_value", C.c_double), ("seconds", C.c_double)]
+
+
+_P = C.c_void_p
+_SIGS = {
+    "mfx_last_error": (C.c_char_p, []),
+    "mfx_version": (C.c_int, []),
+    "mfx_bk3_apply_f64": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, C.c_uint32, _P]),
+    "mfx_bk3_apply_f32": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, C.c_uint32, _P]),
+    "mfx_bk5_apply_f64": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_uint32, _P]),
+    "mfx_bk5_apply_f32": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_uint32, _P]),
+    "mfx_bk3_evec_f64": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P, C.c_uint32, _P]),
+    "mfx_bk5_evec_f64": (C.c_int, [C.c_int, _P, _P, _P, _P, C.c_uint32, _P]),
+    "mfx_gather_f64": (C.c_int, [_P, C.c_uint32, C.c_int, _P, _P, _P]),
+    "mfx_scatter_add_f64": (C.c_int, [_P, C.c_uint32, C.c_int, _P, _P, _P]),
+    "mfx_op_create_box": (C.c_int, [C.POINTER(BoxDesc), C.POINTER(C.c_void_p)]),
+    "mfx_op_create": (C.c_int, [C.POINTER(OpDesc), C.POINTER(C.c_void_p)]),
+    "mfx_op_destroy": (C.c_int, [_P]),
+    "mfx_op_n_dofs": (C.c_uint64, [_P]),
+    "mfx_op_n_cells": (C.c_uint32, [_P]),
+    "mfx_op_get_info": (C.c_int, [_P, C.POINTER(OpInfo)]),
+    "mfx_op_vmult": (C.c_int, [_P, _P, _P, _P]),
+    "mfx_op_vmult_f32": (C.c_int, [_P, _P, _P, _P]),
+    "mfx_op_diagonal": (C.c_int, [_P, _P, _P]),
+    "mfx_op_inverse_diagonal": (C.c_int, [_P, _P, _P]),
+    "mfx_op_rhs": (C.c_int, [_P, C.c_int, _P, _P]),
+    "mfx_op_dof_coordinates": (C.c_int, [_P, _P, _P]),
+    "mfx_cg_solve": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double,
+                               C.POINTER(CgResult), _P]),
+    "mfx_vec_random": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P]),
+    "mfx_vec_zero_constrained": (C.c_int, [_P, _P, _P]),
+    "mfx_vec_dot": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_double), _P]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libmfx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        # torch ships its own HIP runtime; loading it first makes libmfx bind to
+        # that one, so the process holds a single HIP runtime.
+        import torch  # noqa: F401
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                "g.build()'` (or `make -C benchmarks_amd/csrc`). There is no CPU fallback.")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != MFX_OK:
+        raise MfxError(status, lib().mfx_last_error().decode())
+
+
+def exported_names() -> list:
+    return list(_SIGS)

@@ -1,0 +1,447 @@
+// apply.hip -- sum-factorized scalar-Laplacian element kernels for gfx950.
+//
+// One kernel family covers the reference's BK3 (Gauss, interpolation B) and
+// BK5 (GLL collocation, B = I) element operators
+//   v_e = B^T D^T G_e D B u_e
+// (CEED_bp/include/bk3_kokkos_kernel.h:113-382,
+//  CEED_BK/include/kernels/BK5/templated_cuda_kernels.cuh:37-125), with the
+// DoF-map gather (masked entries read as 0) and the scatter-add (masked
+// entries skipped) of CEED_bp/include/bk3_kokkos_kernel.h:113-143, 356-382.
+//
+// Work decomposition (CDNA4, 64-wide wavefronts):
+//   * a workgroup holds EPB elements, one thread per (x,y) quadrature column
+//     (EPB * nq^2 <= 256 threads); each thread keeps its z-column in
+//     registers, so z contractions are register-local and only the x and y
+//     contractions go through LDS (3 LDS cubes per element);
+//   * the map and G are read with the column index fastest, so every load
+//     instruction of a wavefront touches contiguous 8-byte runs (G layout
+//     [cell][6][q^3], x fastest);
+//   * scatter: hardware FP64/FP32 atomic add in L2 (global_atomic_add_f64);
+//   * workgroup b runs on XCD b % 8 (round-robin dispatch); the batch index is
+//     remapped so each XCD walks one contiguous eighth of the cells, keeping
+//     the src/dst lines shared by neighbouring cells inside one XCD's L2.
+#include <type_traits>
+
+#include "mfx_internal.hpp"
+
+namespace mfx {
+namespace {
+
+template <int NQ>
+struct Tile {
+  static constexpr int EPB = (256 / (NQ * NQ)) > 0 ? 256 / (NQ * NQ) : 1;
+  static constexpr int THREADS = EPB * NQ * NQ;
+};
+
+__device__ __forceinline__ uint32_t logical_batch(uint32_t b, uint32_t per_xcd) {
+  return (b & 7u) * per_xcd + (b >> 3);
+}
+
+__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
+
+// Sum over the workgroup; result valid in thread 0.
+template <int THREADS>
+__device__ __forceinline__ double block_sum(double v, double *red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  constexpr int NW = (THREADS + 63) / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NW; ++i) s += red[i];
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// GLL collocation (BK5): NM == NQ, B = I.
+// ---------------------------------------------------------------------------
+template <typename T, int NQ, bool GATHER, bool DOT>
+__global__ void __launch_bounds__(Tile<NQ>::THREADS)
+k_apply_colloc(const T *__restrict__ gD, const T *__restrict__ G,
+               const T *__restrict__ src, T *__restrict__ dst,
+               const uint32_t *__restrict__ map, uint32_t n_cells,
+               uint32_t n_batches, uint32_t per_xcd, double *__restrict__ partials,
+               const int *__restrict__ skip) {
+  constexpr int EPB = Tile<NQ>::EPB, NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
+  __shared__ T sD[NQ * NQ];
+  __shared__ T su[EPB][NQ3];
+  __shared__ T sfx[EPB][NQ3];
+  __shared__ T sfy[EPB][NQ3];
+  __shared__ double red[(Tile<NQ>::THREADS + 63) / 64];
+
+  if (skip && *skip) return;  // converged CG: nothing left to do
+  const uint32_t batch = logical_batch(blockIdx.x, per_xcd);
+  if (batch >= n_batches) {
+    if (DOT && threadIdx.x == 0) partials[blockIdx.x] = 0.0;
+    return;
+  }
+  const int tid = threadIdx.x;
+  const int el = tid / NQ2;
+  const int col = tid - el * NQ2;
+  const int qx = col % NQ, qy = col / NQ;
+  const uint32_t e = batch * EPB + el;
+  const bool active = e < n_cells;
+
+  for (int i = tid; i < NQ * NQ; i += Tile<NQ>::THREADS) sD[i] = gD[i];
+
+  uint32_t idx[NQ];
+  T u[NQ];
+  if (active) {
+    if (GATHER) {
+      const uint32_t *me = map + (size_t)e * NQ3 + col;
+#pragma unroll
+      for (int z = 0; z < NQ; ++z) idx[z] = __builtin_nontemporal_load(me + z * NQ2);
+#pragma unroll
+      for (int z = 0; z < NQ; ++z) u[z] = (idx[z] != kInvalid) ? src[idx[z]] : T(0);
+    } else {
+      const T *se = src + (size_t)e * NQ3 + col;
+#pragma unroll
+      for (int z = 0; z < NQ; ++z) u[z] = se[z * NQ2];
+    }
+#pragma unroll
+    for (int z = 0; z < NQ; ++z) su[el][z * NQ2 + col] = u[z];
+  }
+  __syncthreads();
+
+  T fz[NQ];
+  if (active) {
+    const T *Ge = G + (size_t)e * 6 * NQ3 + col;
+#pragma unroll
+    for (int z = 0; z < NQ; ++z) {
+      const T gxx = __builtin_nontemporal_load(Ge + 0 * NQ3 + z * NQ2);
+      const T gxy = __builtin_nontemporal_load(Ge + 1 * NQ3 + z * NQ2);
+      const T gxz = __builtin_nontemporal_load(Ge + 2 * NQ3 + z * NQ2);
+      const T gyy = __builtin_nontemporal_load(Ge + 3 * NQ3 + z * NQ2);
+      const T gyz = __builtin_nontemporal_load(Ge + 4 * NQ3 + z * NQ2);
+      const T gzz = __builtin_nontemporal_load(Ge + 5 * NQ3 + z * NQ2);
+      T dx = 0, dy = 0, dz = 0;
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) {
+        dx += sD[qx * NQ + a] * su[el][z * NQ2 + qy * NQ + a];
+        dy += sD[qy * NQ + a] * su[el][z * NQ2 + a * NQ + qx];
+        dz += sD[z * NQ + a] * u[a];
+      }
+      sfx[el][z * NQ2 + col] = gxx * dx + gxy * dy + gxz * dz;
+      sfy[el][z * NQ2 + col] = gxy * dx + gyy * dy + gyz * dz;
+      fz[z] = gxz * dx + gyz * dy + gzz * dz;
+    }
+  }
+  __syncthreads();
+
+  double dot = 0.0;
+  if (active) {
+#pragma unroll
+    for (int z = 0; z < NQ; ++z) {
+      T v = 0;
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) {
+        v += sD[a * NQ + qx] * sfx[el][z * NQ2 + qy * NQ + a];
+        v += sD[a * NQ + qy] * sfy[el][z * NQ2 + a * NQ + qx];
+        v += sD[a * NQ + z] * fz[a];
+      }
+      if (DOT) dot += (double)u[z] * (double)v;
+      if (GATHER) {
+        if (idx[z] != kInvalid) atomic_add(dst + idx[z], v);
+      } else {
+        dst[(size_t)e * NQ3 + z * NQ2 + col] = v;
+      }
+    }
+  }
+  if (DOT) {
+    const double s = block_sum<Tile<NQ>::THREADS>(dot, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// General tensor-product element (BK3): NM = p+1 nodes, NQ >= NM quadrature
+// points, interpolation B[q][m] and collocation derivative D[q][k].
+// ---------------------------------------------------------------------------
+template <typename T, int NM, int NQ, bool GATHER, bool DOT>
+__global__ void __launch_bounds__(Tile<NQ>::THREADS)
+k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
+               const T *__restrict__ G, const T *__restrict__ src,
+               T *__restrict__ dst, const uint32_t *__restrict__ map,
+               uint32_t n_cells, uint32_t n_batches, uint32_t per_xcd,
+               double *__restrict__ partials, const int *__restrict__ skip) {
+  constexpr int EPB = Tile<NQ>::EPB, NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
+  constexpr int NM2 = NM * NM, NM3 = NM2 * NM;
+  __shared__ T sB[NQ * NM];
+  __shared__ T sD[NQ * NQ];
+  __shared__ T s0[EPB][NQ3];
+  __shared__ T s1[EPB][NQ3];
+  __shared__ T s2[EPB][NQ3];
+  __shared__ double red[(Tile<NQ>::THREADS + 63) / 64];
+
+  if (skip && *skip) return;  // converged CG: nothing left to do
+  const uint32_t batch = logical_batch(blockIdx.x, per_xcd);
+  if (batch >= n_batches) {
+    if (DOT && threadIdx.x == 0) partials[blockIdx.x] = 0.0;
+    return;
+  }
+  const int tid = threadIdx.x;
+  const int el = tid / NQ2;
+  const int col = tid - el * NQ2;
+  const int qx = col % NQ, qy = col / NQ;
+  const uint32_t e = batch * EPB + el;
+  const bool active = e < n_cells;
+  const bool node_col = active && qx < NM && qy < NM;
+
+  for (int i = tid; i < NQ * NM; i += Tile<NQ>::THREADS) sB[i] = gB[i];
+  for (int i = tid; i < NQ * NQ; i += Tile<NQ>::THREADS) sD[i] = gD[i];
+
+  // 1. gather the (x,y) node column
+  uint32_t idx[NM];
+  T uin[NM];
+  if (node_col) {
+    const int l0 = qy * NM + qx;
+    if (GATHER) {
+      const uint32_t *me = map + (size_t)e * NM3 + l0;
+#pragma unroll
+      for (int z = 0; z < NM; ++z) idx[z] = __builtin_nontemporal_load(me + z * NM2);
+#pragma unroll
+      for (int z = 0; z < NM; ++z) uin[z] = (idx[z] != kInvalid) ? src[idx[z]] : T(0);
+    } else {
+      const T *se = src + (size_t)e * NM3 + l0;
+#pragma unroll
+      for (int z = 0; z < NM; ++z) uin[z] = se[z * NM2];
+    }
+#pragma unroll
+    for (int z = 0; z < NM; ++z) s0[el][z * NM2 + l0] = uin[z];
+  }
+  __syncthreads();
+
+  // 2. interpolate along x: s1[z][y][qx], z,y < NM
+  if (active && qy < NM) {
+#pragma unroll
+    for (int z = 0; z < NM; ++z) {
+      T t = 0;
+#pragma unroll
+      for (int a = 0; a < NM; ++a) t += sB[qx * NM + a] * s0[el][z * NM2 + qy * NM + a];
+      s1[el][z * NM * NQ + qy * NQ + qx] = t;
+    }
+  }
+  __syncthreads();
+
+  // 3./4. interpolate along y (LDS) and z (registers), then 5. gradient
+  T uq[NQ];
+  if (active) {
+    T t2[NM];
+#pragma unroll
+    for (int z = 0; z < NM; ++z) {
+      T t = 0;
+#pragma unroll
+      for (int b = 0; b < NM; ++b) t += sB[qy * NM + b] * s1[el][z * NM * NQ + b * NQ + qx];
+      t2[z] = t;
+    }
+#pragma unroll
+    for (int zq = 0; zq < NQ; ++zq) {
+      T t = 0;
+#pragma unroll
+      for (int c = 0; c < NM; ++c) t += sB[zq * NM + c] * t2[c];
+      uq[zq] = t;
+      s2[el][zq * NQ2 + col] = t;
+    }
+  }
+  __syncthreads();
+
+  T fz[NQ];
+  if (active) {
+    const T *Ge = G + (size_t)e * 6 * NQ3 + col;
+#pragma unroll
+    for (int z = 0; z < NQ; ++z) {
+      const T gxx = __builtin_nontemporal_load(Ge + 0 * NQ3 + z * NQ2);
+      const T gxy = __builtin_nontemporal_load(Ge + 1 * NQ3 + z * NQ2);
+      const T gxz = __builtin_nontemporal_load(Ge + 2 * NQ3 + z * NQ2);
+      const T gyy = __builtin_nontemporal_load(Ge + 3 * NQ3 + z * NQ2);
+      const T gyz = __builtin_nontemporal_load(Ge + 4 * NQ3 + z * NQ2);
+      const T gzz = __builtin_nontemporal_load(Ge + 5 * NQ3 + z * NQ2);
+      T dx = 0, dy = 0, dz = 0;
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) {
+        dx += sD[qx * NQ + a] * s2[el][z * NQ2 + qy * NQ + a];
+        dy += sD[qy * NQ + a] * s2[el][z * NQ2 + a * NQ + qx];
+        dz += sD[z * NQ + a] * uq[a];
+      }
+      s0[el][z * NQ2 + col] = gxx * dx + gxy * dy + gxz * dz;
+      s1[el][z * NQ2 + col] = gxy * dx + gyy * dy + gyz * dz;
+      fz[z] = gxz * dx + gyz * dy + gzz * dz;
+    }
+  }
+  __syncthreads();
+
+  // 6. transposed gradient, 7. transposed interpolation along z (registers)
+  if (active) {
+    T w[NQ];
+#pragma unroll
+    for (int z = 0; z < NQ; ++z) {
+      T v = 0;
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) {
+        v += sD[a * NQ + qx] * s0[el][z * NQ2 + qy * NQ + a];
+        v += sD[a * NQ + qy] * s1[el][z * NQ2 + a * NQ + qx];
+        v += sD[a * NQ + z] * fz[a];
+      }
+      w[z] = v;
+    }
+#pragma unroll
+    for (int zn = 0; zn < NM; ++zn) {
+      T t = 0;
+#pragma unroll
+      for (int c = 0; c < NQ; ++c) t += sB[c * NM + zn] * w[c];
+      s2[el][zn * NQ2 + col] = t;
+    }
+  }
+  __syncthreads();
+
+  // 8. transposed interpolation along y: s0[zn][yn][qx]
+  if (active && qy < NM) {
+#pragma unroll
+    for (int zn = 0; zn < NM; ++zn) {
+      T t = 0;
+#pragma unroll
+      for (int b = 0; b < NQ; ++b) t += sB[b * NM + qy] * s2[el][zn * NQ2 + b * NQ + qx];
+      s0[el][zn * NM * NQ + qy * NQ + qx] = t;
+    }
+  }
+  __syncthreads();
+
+  // 9. transposed interpolation along x and scatter
+  double dot = 0.0;
+  if (node_col) {
+#pragma unroll
+    for (int zn = 0; zn < NM; ++zn) {
+      T v = 0;
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) v += sB[a * NM + qx] * s0[el][zn * NM * NQ + qy * NQ + a];
+      if (DOT) dot += (double)uin[zn] * (double)v;
+      if (GATHER) {
+        if (idx[zn] != kInvalid) atomic_add(dst + idx[zn], v);
+      } else {
+        dst[(size_t)e * NM3 + zn * NM2 + qy * NM + qx] = v;
+      }
+    }
+  }
+  if (DOT) {
+    const double s = block_sum<Tile<NQ>::THREADS>(dot, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  }
+}
+
+template <int NQ>
+uint32_t n_blocks_for(uint32_t n_cells) {
+  const uint32_t nb = (n_cells + Tile<NQ>::EPB - 1) / Tile<NQ>::EPB;
+  return 8u * ((nb + 7u) / 8u);
+}
+
+template <typename T, int NM, int NQ, bool COLLOC, bool GATHER>
+void launch_t(const T *B, const T *D, const T *G, const T *src, T *dst,
+              const uint32_t *map, uint32_t n_cells, hipStream_t s,
+              double *partials, const int *skip) {
+  if (n_cells == 0) return;
+  const uint32_t nb = (n_cells + Tile<NQ>::EPB - 1) / Tile<NQ>::EPB;
+  const uint32_t per = (nb + 7u) / 8u;
+  const dim3 grid(8u * per), block(Tile<NQ>::THREADS);
+  if constexpr (COLLOC) {
+    static_assert(NM == NQ, "collocation needs nm == nq");
+    if (partials) {
+      if constexpr (std::is_same<T, double>::value && GATHER)
+        hipLaunchKernelGGL((k_apply_colloc<T, NQ, GATHER, true>), grid, block, 0, s,
+                           D, G, src, dst, map, n_cells, nb, per, partials, skip);
+    } else {
+      hipLaunchKernelGGL((k_apply_colloc<T, NQ, GATHER, false>), grid, block, 0, s,
+                         D, G, src, dst, map, n_cells, nb, per, nullptr, nullptr);
+    }
+  } else {
+    if (partials) {
+      if constexpr (std::is_same<T, double>::value && GATHER)
+        hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, true>), grid, block, 0, s,
+                           B, D, G, src, dst, map, n_cells, nb, per, partials, skip);
+    } else {
+      hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, false>), grid, block, 0, s,
+                         B, D, G, src, dst, map, n_cells, nb, per, nullptr, nullptr);
+    }
+  }
+  MFX_HIP(hipGetLastError());
+}
+
+// Runtime (p, nq, collocation) -> template instance.  Supported: p = 1..8;
+// collocation nq = p+1; interpolation nq = p+1 or p+2.
+template <typename T, bool GATHER>
+void dispatch(int p, int nq, bool colloc, const T *B, const T *D, const T *G,
+              const T *src, T *dst, const uint32_t *map, uint32_t n,
+              hipStream_t s, double *partials, const int *skip) {
+#define MFX_CASE_C(P)                                                         \
+  case P:                                                                     \
+    return launch_t<T, P + 1, P + 1, true, GATHER>(B, D, G, src, dst, map, n, s, partials, skip);
+#define MFX_CASE_I(P)                                                         \
+  case P:                                                                     \
+    if (nq == P + 1)                                                          \
+      return launch_t<T, P + 1, P + 1, false, GATHER>(B, D, G, src, dst, map, n, s, partials, skip); \
+    if (nq == P + 2)                                                          \
+      return launch_t<T, P + 1, P + 2, false, GATHER>(B, D, G, src, dst, map, n, s, partials, skip); \
+    break;
+  if (colloc) {
+    MFX_REQUIRE(nq == p + 1, "collocation requires nq == p+1");
+    switch (p) { MFX_CASE_C(1) MFX_CASE_C(2) MFX_CASE_C(3) MFX_CASE_C(4)
+                 MFX_CASE_C(5) MFX_CASE_C(6) MFX_CASE_C(7) MFX_CASE_C(8) default: break; }
+  } else {
+    switch (p) { MFX_CASE_I(1) MFX_CASE_I(2) MFX_CASE_I(3) MFX_CASE_I(4)
+                 MFX_CASE_I(5) MFX_CASE_I(6) MFX_CASE_I(7) MFX_CASE_I(8) default: break; }
+  }
+#undef MFX_CASE_C
+#undef MFX_CASE_I
+  throw Error{MFX_ERR_UNSUPPORTED,
+              "unsupported (degree, n_q_points_1d) = (" + std::to_string(p) + ", " +
+                  std::to_string(nq) + ")"};
+}
+
+}  // namespace
+
+template <typename T>
+void launch_apply(int p, int nq, bool colloc, const T *B, const T *D, const T *G,
+                  const T *src, T *dst, const uint32_t *map, uint32_t n_cells,
+                  hipStream_t s, double *dot_partials, const int *skip) {
+  dispatch<T, true>(p, nq, colloc, B, D, G, src, dst, map, n_cells, s, dot_partials, skip);
+}
+template <typename T>
+void launch_evec(int p, int nq, bool colloc, const T *B, const T *D, const T *G,
+                 const T *in, T *out, uint32_t n_cells, hipStream_t s) {
+  dispatch<T, false>(p, nq, colloc, B, D, G, in, out, nullptr, n_cells, s, nullptr, nullptr);
+}
+
+template void launch_apply<double>(int, int, bool, const double *, const double *,
+                                   const double *, const double *, double *,
+                                   const uint32_t *, uint32_t, hipStream_t, double *,
+                                   const int *);
+template void launch_apply<float>(int, int, bool, const float *, const float *,
+                                  const float *, const float *, float *,
+                                  const uint32_t *, uint32_t, hipStream_t, double *,
+                                   const int *);
+template void launch_evec<double>(int, int, bool, const double *, const double *,
+                                  const double *, const double *, double *, uint32_t,
+                                  hipStream_t);
+
+uint32_t apply_n_blocks(int p, int nq, bool colloc, uint32_t n_cells) {
+  (void)colloc;
+  (void)p;
+  switch (nq) {
+    case 2: return n_blocks_for<2>(n_cells);
+    case 3: return n_blocks_for<3>(n_cells);
+    case 4: return n_blocks_for<4>(n_cells);
+    case 5: return n_blocks_for<5>(n_cells);
+    case 6: return n_blocks_for<6>(n_cells);
+    case 7: return n_blocks_for<7>(n_cells);
+    case 8: return n_blocks_for<8>(n_cells);
+    case 9: return n_blocks_for<9>(n_cells);
+    case 10: return n_blocks_for<10>(n_cells);
+    default: return 0;
+  }
+}
+
+}  // namespace mfx

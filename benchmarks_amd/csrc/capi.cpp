@@ -1,0 +1,472 @@
+// capi.cpp -- extern "C" entry points of libmfx (declared in include/mfx.h).
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "cg_state.hpp"
+#include "mfx_internal.hpp"
+
+namespace mfx {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+template <typename F>
+static mfx_status guard(F &&f) {
+  try {
+    f();
+    return MFX_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return MFX_ERR_OOM;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return MFX_ERR_HIP;
+  }
+}
+
+// Owning device buffer.
+template <typename T>
+struct DBuf {
+  T *ptr = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { reset(); }
+  void reset() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    reset();
+    if (count == 0) return;
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess)
+      throw Error{e == hipErrorOutOfMemory ? MFX_ERR_OOM : MFX_ERR_HIP,
+                  "hipMalloc(" + std::to_string(count * sizeof(T)) + " B): " +
+                      hipGetErrorString(e)};
+    ptr = static_cast<T *>(p);
+    n = count;
+  }
+  void upload(const T *host, size_t count) {
+    alloc(count);
+    if (count) MFX_HIP(hipMemcpy(ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+};
+
+}  // namespace mfx
+
+using namespace mfx;
+
+struct mfx_op {
+  int p = 0, nq = 0, quad = 0, geometry = 0;
+  bool colloc = false;
+  uint32_t E = 0;
+  uint64_t N = 0;
+  uint64_t n_cons = 0;
+  DBuf<uint32_t> map, cons;
+  DBuf<double> G, B, D, xq, wq, support, cellX;
+  DBuf<float> G32, B32, D32;
+  // CG scratch, allocated on first use
+  DBuf<double> r, pdir, ap, part;
+  DBuf<CgState> st;
+  int *h_done = nullptr;  // pinned
+  ~mfx_op() {
+    if (h_done) (void)hipHostFree(h_done);
+  }
+};
+
+static hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+static void finish_setup(mfx_op *op, int with_fp32, hipStream_t s) {
+  const uint64_t nq3 = (uint64_t)op->nq * op->nq * op->nq;
+  if (op->geometry == MFX_GEOM_STORED) {
+    op->G.alloc((uint64_t)op->E * 6 * nq3);
+    launch_geometry(op->E, op->cellX.ptr, op->nq, op->xq.ptr, op->wq.ptr, op->G.ptr, s);
+  }
+  if (with_fp32) {
+    if (op->geometry == MFX_GEOM_STORED) {
+      op->G32.alloc(op->G.n);
+      launch_to_float(op->G.ptr, op->G32.ptr, op->G.n, s);
+    }
+    op->B32.alloc(op->B.n);
+    launch_to_float(op->B.ptr, op->B32.ptr, op->B.n, s);
+    op->D32.alloc(op->D.n);
+    launch_to_float(op->D.ptr, op->D32.ptr, op->D.n, s);
+  }
+  MFX_HIP(hipStreamSynchronize(s));
+}
+
+static void common_init(mfx_op *op, int p, int nq, int quad, int geometry) {
+  MFX_REQUIRE(p >= 1 && p <= 8, "degree must be in 1..8");
+  MFX_REQUIRE(quad == MFX_QUAD_GAUSS || quad == MFX_QUAD_GLL, "unknown quadrature");
+  MFX_REQUIRE(nq == p + 1 || nq == p + 2, "n_q_points_1d must be degree+1 or degree+2");
+  MFX_REQUIRE(geometry == MFX_GEOM_STORED, "only stored geometry is implemented");
+  op->p = p;
+  op->nq = nq;
+  op->quad = quad;
+  op->geometry = geometry;
+  op->colloc = (quad == MFX_QUAD_GLL && nq == p + 1);
+  std::vector<double> B((size_t)nq * (p + 1)), D((size_t)nq * nq), xq(nq), wq(nq), sup(p + 1);
+  basis_tables(p, nq, quad, B.data(), D.data(), xq.data(), wq.data(), sup.data());
+  op->B.upload(B.data(), B.size());
+  op->D.upload(D.data(), D.size());
+  op->xq.upload(xq.data(), xq.size());
+  op->wq.upload(wq.data(), wq.size());
+  op->support.upload(sup.data(), sup.size());
+}
+
+extern "C" {
+
+const char *mfx_last_error(void) { return g_last_error.c_str(); }
+int mfx_version(void) { return 100; }
+
+// ---------------------------------------------------------------- kernels
+mfx_status mfx_bk3_apply_f64(int p, int nq, const double *B, const double *D, const double *G,
+                             const double *src, double *dst, const uint32_t *map,
+                             uint32_t n_cells, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(B && D && G && src && dst && map, "null pointer argument");
+    launch_apply<double>(p, nq, false, B, D, G, src, dst, map, n_cells, as_stream(stream));
+  });
+}
+mfx_status mfx_bk3_apply_f32(int p, int nq, const float *B, const float *D, const float *G,
+                             const float *src, float *dst, const uint32_t *map,
+                             uint32_t n_cells, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(B && D && G && src && dst && map, "null pointer argument");
+    launch_apply<float>(p, nq, false, B, D, G, src, dst, map, n_cells, as_stream(stream));
+  });
+}
+mfx_status mfx_bk5_apply_f64(int p, const double *D, const double *G, const double *src,
+                             double *dst, const uint32_t *map, uint32_t n_cells, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(D && G && src && dst && map, "null pointer argument");
+    launch_apply<double>(p, p + 1, true, nullptr, D, G, src, dst, map, n_cells,
+                         as_stream(stream));
+  });
+}
+mfx_status mfx_bk5_apply_f32(int p, const float *D, const float *G, const float *src,
+                             float *dst, const uint32_t *map, uint32_t n_cells, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(D && G && src && dst && map, "null pointer argument");
+    launch_apply<float>(p, p + 1, true, nullptr, D, G, src, dst, map, n_cells,
+                        as_stream(stream));
+  });
+}
+mfx_status mfx_bk3_evec_f64(int p, int nq, const double *B, const double *D, const double *G,
+                            const double *in, double *out, uint32_t n_cells, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(B && D && G && in && out, "null pointer argument");
+    launch_evec<double>(p, nq, false, B, D, G, in, out, n_cells, as_stream(stream));
+  });
+}
+mfx_status mfx_bk5_evec_f64(int p, const double *D, const double *G, const double *in,
+                            double *out, uint32_t n_cells, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(D && G && in && out, "null pointer argument");
+    launch_evec<double>(p, p + 1, true, nullptr, D, G, in, out, n_cells, as_stream(stream));
+  });
+}
+
+mfx_status mfx_gather_f64(const uint32_t *map, uint32_t n_cells, int nodes_per_cell,
+                          const double *src, double *evec, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(map && src && evec, "null pointer argument");
+    MFX_REQUIRE(nodes_per_cell > 0, "nodes_per_cell must be positive");
+    launch_gather(map, (uint64_t)n_cells * nodes_per_cell, src, evec, as_stream(stream));
+  });
+}
+mfx_status mfx_scatter_add_f64(const uint32_t *map, uint32_t n_cells, int nodes_per_cell,
+                               const double *evec, double *dst, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(map && evec && dst, "null pointer argument");
+    MFX_REQUIRE(nodes_per_cell > 0, "nodes_per_cell must be positive");
+    launch_scatter_add(map, (uint64_t)n_cells * nodes_per_cell, evec, dst, as_stream(stream));
+  });
+}
+
+// ---------------------------------------------------------------- operator
+mfx_status mfx_op_create_box(const mfx_box_desc *d, mfx_op **out) {
+  return guard([&] {
+    MFX_REQUIRE(d && out, "null pointer argument");
+    *out = nullptr;
+    auto op = std::make_unique<mfx_op>();
+    common_init(op.get(), d->degree, d->n_q_points_1d, d->quadrature, d->geometry);
+    const int p = d->degree;
+    uint64_t E = 1, N = 1, Ninner = 1;
+    for (int k = 0; k < 3; ++k) {
+      MFX_REQUIRE(d->n_cells[k] >= 1, "n_cells must be >= 1");
+      MFX_REQUIRE(d->hi[k] > d->lo[k], "box must have hi > lo");
+      E *= d->n_cells[k];
+      const uint64_t nk = (uint64_t)d->n_cells[k] * p + 1;
+      N *= nk;
+      Ninner *= nk - 2;
+    }
+    MFX_REQUIRE(E <= 0xFFFFFFFFull / 2, "too many cells");
+    MFX_REQUIRE(N < 0xFFFFFFFFull, "too many DoFs for 32-bit DoF indices");
+    op->E = (uint32_t)E;
+    op->N = N;
+    BoxGeom box;
+    for (int k = 0; k < 3; ++k) {
+      box.n[k] = d->n_cells[k];
+      box.lo[k] = d->lo[k];
+      box.hi[k] = d->hi[k];
+    }
+    box.deform = d->deform;
+    // constrained DoFs: the boundary nodes of the structured node grid
+    const uint64_t Nx = (uint64_t)d->n_cells[0] * p + 1, Ny = (uint64_t)d->n_cells[1] * p + 1,
+                   Nz = (uint64_t)d->n_cells[2] * p + 1;
+    std::vector<uint32_t> cons;
+    cons.reserve(N - Ninner);
+    for (uint64_t z = 0; z < Nz; ++z)
+      for (uint64_t y = 0; y < Ny; ++y) {
+        const bool face = (z == 0 || z == Nz - 1 || y == 0 || y == Ny - 1);
+        const uint64_t row = Nx * (y + Ny * z);
+        if (face) {
+          for (uint64_t x = 0; x < Nx; ++x) cons.push_back((uint32_t)(row + x));
+        } else {
+          cons.push_back((uint32_t)row);
+          cons.push_back((uint32_t)(row + Nx - 1));
+        }
+      }
+    op->n_cons = cons.size();
+    op->cons.upload(cons.data(), cons.size());
+    hipStream_t s = nullptr;
+    op->cellX.alloc(E * 24);
+    launch_box_cell_vertices(box, op->cellX.ptr, s);
+    op->map.alloc(E * (uint64_t)(p + 1) * (p + 1) * (p + 1));
+    launch_box_map(box, p, op->map.ptr, s);
+    finish_setup(op.get(), d->with_fp32, s);
+    *out = op.release();
+  });
+}
+
+mfx_status mfx_op_create(const mfx_op_desc *d, mfx_op **out) {
+  return guard([&] {
+    MFX_REQUIRE(d && out, "null pointer argument");
+    *out = nullptr;
+    MFX_REQUIRE(d->map && d->cell_vertices, "map and cell_vertices are required");
+    MFX_REQUIRE(d->n_constrained == 0 || d->constrained, "constrained list missing");
+    MFX_REQUIRE(d->n_dofs < 0xFFFFFFFFull, "too many DoFs for 32-bit DoF indices");
+    auto op = std::make_unique<mfx_op>();
+    common_init(op.get(), d->degree, d->n_q_points_1d, d->quadrature, d->geometry);
+    const uint64_t nm3 = (uint64_t)(d->degree + 1) * (d->degree + 1) * (d->degree + 1);
+    const uint64_t nmap = (uint64_t)d->n_cells * nm3;
+    for (uint64_t i = 0; i < nmap; ++i)
+      MFX_REQUIRE(d->map[i] == MFX_INVALID_DOF || d->map[i] < d->n_dofs,
+                  "map entry out of range at " + std::to_string(i));
+    for (uint64_t i = 0; i < d->n_constrained; ++i)
+      MFX_REQUIRE(d->constrained[i] < d->n_dofs, "constrained index out of range");
+    op->E = d->n_cells;
+    op->N = d->n_dofs;
+    op->n_cons = d->n_constrained;
+    op->map.upload(d->map, nmap);
+    op->cellX.upload(d->cell_vertices, (uint64_t)d->n_cells * 24);
+    op->cons.upload(d->constrained, d->n_constrained);
+    finish_setup(op.get(), d->with_fp32, nullptr);
+    *out = op.release();
+  });
+}
+
+mfx_status mfx_op_destroy(mfx_op *op) {
+  return guard([&] {
+    if (op) {
+      (void)hipDeviceSynchronize();
+      delete op;
+    }
+  });
+}
+
+uint64_t mfx_op_n_dofs(const mfx_op *op) { return op ? op->N : 0; }
+uint32_t mfx_op_n_cells(const mfx_op *op) { return op ? op->E : 0; }
+
+mfx_status mfx_op_get_info(const mfx_op *op, mfx_op_info *info) {
+  return guard([&] {
+    MFX_REQUIRE(op && info, "null pointer argument");
+    info->degree = op->p;
+    info->n_q_points_1d = op->nq;
+    info->quadrature = op->quad;
+    info->geometry = op->geometry;
+    info->n_cells = op->E;
+    info->n_dofs = op->N;
+    info->n_constrained = op->n_cons;
+    info->map = op->map.ptr;
+    info->G = op->G.ptr;
+    info->B = op->B.ptr;
+    info->D = op->D.ptr;
+    info->cell_vertices = op->cellX.ptr;
+    info->constrained = op->cons.ptr;
+    info->xq = op->xq.ptr;
+    info->wq = op->wq.ptr;
+  });
+}
+
+mfx_status mfx_op_vmult(mfx_op *op, double *dst, const double *src, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(op && dst && src, "null pointer argument");
+    MFX_REQUIRE(dst != src, "vmult: src and dst must not alias");
+    hipStream_t s = as_stream(stream);
+    launch_fill(dst, 0.0, op->N, s);
+    launch_apply<double>(op->p, op->nq, op->colloc, op->B.ptr, op->D.ptr, op->G.ptr, src, dst,
+                         op->map.ptr, op->E, s);
+    launch_copy_listed<double>(dst, src, op->cons.ptr, op->n_cons, s);
+  });
+}
+
+mfx_status mfx_op_vmult_f32(mfx_op *op, float *dst, const float *src, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(op && dst && src, "null pointer argument");
+    MFX_REQUIRE(dst != src, "vmult: src and dst must not alias");
+    MFX_REQUIRE(op->D32.ptr, "operator created without with_fp32");
+    hipStream_t s = as_stream(stream);
+    launch_fill_f32(dst, 0.0f, op->N, s);
+    launch_apply<float>(op->p, op->nq, op->colloc, op->B32.ptr, op->D32.ptr, op->G32.ptr, src,
+                        dst, op->map.ptr, op->E, s);
+    launch_copy_listed<float>(dst, src, op->cons.ptr, op->n_cons, s);
+  });
+}
+
+mfx_status mfx_op_diagonal(mfx_op *op, double *diag, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(op && diag, "null pointer argument");
+    hipStream_t s = as_stream(stream);
+    launch_fill(diag, 0.0, op->N, s);
+    launch_diagonal(op->E, op->p, op->nq, op->B.ptr, op->D.ptr, op->G.ptr, op->map.ptr, diag, s);
+    launch_set_listed(diag, 1.0, op->cons.ptr, op->n_cons, s);
+  });
+}
+
+mfx_status mfx_op_inverse_diagonal(mfx_op *op, double *inv, void *stream) {
+  mfx_status st = mfx_op_diagonal(op, inv, stream);
+  if (st != MFX_OK) return st;
+  return guard([&] { launch_invert_diagonal(inv, op->N, as_stream(stream)); });
+}
+
+mfx_status mfx_op_rhs(mfx_op *op, int f_kind, double *b, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(op && b, "null pointer argument");
+    MFX_REQUIRE(f_kind == 0 || f_kind == 1, "f_kind must be 0 or 1");
+    hipStream_t s = as_stream(stream);
+    launch_fill(b, 0.0, op->N, s);
+    launch_rhs(op->E, op->p, op->nq, op->B.ptr, op->xq.ptr, op->wq.ptr, op->cellX.ptr,
+               op->map.ptr, f_kind, b, s);
+  });
+}
+
+mfx_status mfx_op_dof_coordinates(mfx_op *op, double *coords, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(op && coords, "null pointer argument");
+    launch_dof_coordinates(op->E, op->p, op->support.ptr, op->cellX.ptr, op->map.ptr, coords,
+                           as_stream(stream));
+  });
+}
+
+// ---------------------------------------------------------------- CG
+mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b, const double *inv_diag,
+                        int max_steps, double abs_tol, double rel_tol, mfx_cg_result *res,
+                        void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(op && x && b && res, "null pointer argument");
+    MFX_REQUIRE(max_steps >= 0, "max_steps must be >= 0");
+    hipStream_t s = as_stream(stream);
+    const uint64_t N = op->N;
+    const uint32_t nb_apply = apply_n_blocks(op->p, op->nq, op->colloc, op->E);
+    const uint32_t nb_cons = cg_cons_blocks(op->n_cons);
+    const uint32_t nb_vec = cg_vec_blocks(N);
+    const size_t npart = std::max<size_t>((size_t)nb_apply + nb_cons, 2 * (size_t)nb_vec);
+    if (op->r.n != N) {
+      op->r.alloc(N);
+      op->pdir.alloc(N);
+      op->ap.alloc(N);
+    }
+    if (op->part.n < npart) op->part.alloc(npart);
+    if (!op->st.ptr) op->st.alloc(1);
+    if (!op->h_done) MFX_HIP(hipHostMalloc(reinterpret_cast<void **>(&op->h_done), sizeof(int)));
+    CgState init{};
+    MFX_HIP(hipMemcpyAsync(op->st.ptr, &init, sizeof(CgState), hipMemcpyHostToDevice, s));
+
+    const auto t0 = std::chrono::steady_clock::now();
+    // r = A x
+    launch_fill(op->r.ptr, 0.0, N, s);
+    launch_apply<double>(op->p, op->nq, op->colloc, op->B.ptr, op->D.ptr, op->G.ptr, x,
+                         op->r.ptr, op->map.ptr, op->E, s);
+    launch_copy_listed<double>(op->r.ptr, x, op->cons.ptr, op->n_cons, s);
+    cg_init(op->st.ptr, b, op->r.ptr, inv_diag, N, op->part.ptr, abs_tol, rel_tol, max_steps, s);
+    const int *done = &op->st.ptr->done;
+    int batch = 4;
+    for (int it = 0;;) {
+      for (int k = 0; k < batch; ++k, ++it) {
+        cg_direction(op->st.ptr, op->r.ptr, inv_diag, op->pdir.ptr, op->ap.ptr, N, s);
+        launch_apply<double>(op->p, op->nq, op->colloc, op->B.ptr, op->D.ptr, op->G.ptr,
+                             op->pdir.ptr, op->ap.ptr, op->map.ptr, op->E, s, op->part.ptr,
+                             done);
+        cg_constrained(op->st.ptr, op->cons.ptr, op->n_cons, op->pdir.ptr, op->ap.ptr,
+                       op->part.ptr + nb_apply, s);
+        cg_alpha(op->st.ptr, op->part.ptr, nb_apply + nb_cons, s);
+        cg_update(op->st.ptr, x, op->r.ptr, op->pdir.ptr, op->ap.ptr, inv_diag, N,
+                  op->part.ptr, s);
+        cg_check(op->st.ptr, op->part.ptr, nb_vec, s);
+      }
+      MFX_HIP(hipGetLastError());
+      MFX_HIP(hipMemcpyAsync(op->h_done, done, sizeof(int), hipMemcpyDeviceToHost, s));
+      MFX_HIP(hipStreamSynchronize(s));
+      if (*op->h_done) break;
+      if (it > max_steps + 8) break;  // device enforces max_steps; this bounds the host loop
+      batch = std::min(batch * 2, 64);
+    }
+    CgState fin{};
+    MFX_HIP(hipMemcpyAsync(&fin, op->st.ptr, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    MFX_HIP(hipStreamSynchronize(s));
+    const auto t1 = std::chrono::steady_clock::now();
+    res->steps = fin.steps;
+    res->converged = fin.done == 1;
+    res->initial_value = fin.initial;
+    res->last_value = fin.last;
+    res->seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (fin.done != 1)
+      throw Error{fin.breakdown ? MFX_ERR_BREAKDOWN : MFX_ERR_NOT_CONVERGED,
+                  "CG did not converge: steps=" + std::to_string(fin.steps) +
+                      " residual=" + std::to_string(fin.last)};
+  });
+}
+
+// ---------------------------------------------------------------- vectors
+mfx_status mfx_vec_random(double *x, uint64_t n, uint64_t seed, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(x || n == 0, "null pointer argument");
+    launch_random(x, n, seed, as_stream(stream));
+  });
+}
+
+mfx_status mfx_vec_zero_constrained(mfx_op *op, double *x, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(op && x, "null pointer argument");
+    launch_set_listed(x, 0.0, op->cons.ptr, op->n_cons, as_stream(stream));
+  });
+}
+
+mfx_status mfx_vec_dot(const double *x, const double *y, uint64_t n, double *host_result,
+                       void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(x && y && host_result, "null pointer argument");
+    hipStream_t s = as_stream(stream);
+    DBuf<double> part;
+    part.alloc(dot_n_blocks(n) + 1);
+    launch_dot(x, y, n, part.ptr, part.ptr + dot_n_blocks(n), s);
+    MFX_HIP(hipMemcpyAsync(host_result, part.ptr + dot_n_blocks(n), sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    MFX_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,290 @@
+"""Host-side mirror of the reference's operator interface on top of libmfx.
+
+Names and semantics follow the matrix concept deal.II's SolverCG consumes
+(CEED_bp/include/portable_laplace_operator.h:17-96) and the solver pieces the
+reference drivers use (SolverCG, ReductionControl, PreconditionIdentity,
+DiagonalMatrix; CEED_bp/src/bp3.cc:264-288, bp5_kokkos/benchmark.cc:336-378).
+Vectors are torch tensors on the GPU (torch is the device-memory and stream
+plumbing only); every operation runs in libmfx's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+GAUSS, GLL = _lib.QUAD_GAUSS, _lib.QUAD_GLL
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: torch.Tensor):
+    if not t.is_cuda:
+        raise ValueError("libmfx works on device vectors; got a host tensor")
+    if not t.is_contiguous():
+        raise ValueError("vectors must be contiguous")
+    return C.c_void_p(t.data_ptr())
+
+
+class NoConvergence(RuntimeError):
+    """deal.II SolverControl::NoConvergence."""
+
+    def __init__(self, last_step: int, last_residual: float):
+        super().__init__(f"Iterative method reported convergence failure in step {last_step}. "
+                         f"The residual in the last step was {last_residual}.")
+        self.last_step = last_step
+        self.last_residual = last_residual
+
+
+class LaplaceOperator:
+    """Matrix-free scalar Laplacian on a (deformed) structured hex mesh.
+
+    Mirrors Portable::LaplaceOperator<dim, fe_degree, nq, number>
+    (CEED_bp/include/portable_laplace_operator.h:17-96): vmult, Tvmult, m, n,
+    initialize_dof_vector; set-up by degree, quadrature and mesh as in the
+    LaplaceProblem drivers (CEED_bp/src/bp3.cc:129-182)."""
+
+    def __init__(self, degree: int, n_q_points_1d: int, quadrature: int = GAUSS,
+                 n_cells=(8, 8, 8), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                 deform: float = 0.0, with_fp32: bool = False, _handle=None):
+        self._h = None
+        if _handle is not None:
+            self._h = _handle
+        else:
+            if isinstance(n_cells, int):
+                n_cells = (n_cells,) * 3
+            d = _lib.BoxDesc()
+            d.degree, d.n_q_points_1d, d.quadrature = degree, n_q_points_1d, quadrature
+            d.geometry = _lib.GEOM_STORED
+            for k in range(3):
+                d.n_cells[k] = n_cells[k]
+                d.lo[k] = lo[k]
+                d.hi[k] = hi[k]
+            d.deform = deform
+            d.with_fp32 = int(with_fp32)
+            h = C.c_void_p()
+            check(lib().mfx_op_create_box(C.byref(d), C.byref(h)))
+            self._h = h
+        info = _lib.OpInfo()
+        check(lib().mfx_op_get_info(self._h, C.byref(info)))
+        self.info = info
+        self.degree = info.degree
+        self.n_q_points_1d = info.n_q_points_1d
+        self.quadrature = info.quadrature
+
+    @classmethod
+    def from_arrays(cls, degree, n_q_points_1d, quadrature, map_, cell_vertices, constrained,
+                    n_dofs, with_fp32=False):
+        """General (unstructured) creation from host numpy arrays: the DoF map
+        [n_cells][(p+1)^3] (0xFFFFFFFF = constrained), the 8 vertices of each
+        cell and the list of constrained DoFs."""
+        import numpy as np
+        m = np.ascontiguousarray(map_, dtype=np.uint32)
+        cv = np.ascontiguousarray(cell_vertices, dtype=np.float64)
+        cons = np.ascontiguousarray(constrained, dtype=np.uint32)
+        nm3 = (degree + 1) ** 3
+        d = _lib.OpDesc()
+        d.degree, d.n_q_points_1d, d.quadrature = degree, n_q_points_1d, quadrature
+        d.geometry = _lib.GEOM_STORED
+        d.n_cells = m.size // nm3
+        d.n_dofs = n_dofs
+        d.map = m.ctypes.data
+        d.cell_vertices = cv.ctypes.data
+        d.constrained = cons.ctypes.data if cons.size else None
+        d.n_constrained = cons.size
+        d.with_fp32 = int(with_fp32)
+        h = C.c_void_p()
+        check(lib().mfx_op_create(C.byref(d), C.byref(h)))
+        return cls(degree, n_q_points_1d, quadrature, _handle=h)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and lib is not None:
+            try:
+                lib().mfx_op_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    # --- matrix concept -------------------------------------------------
+    def m(self) -> int:
+        return int(lib().mfx_op_n_dofs(self._h))
+
+    def n(self) -> int:
+        return self.m()
+
+    def n_cells(self) -> int:
+        return int(lib().mfx_op_n_cells(self._h))
+
+    def initialize_dof_vector(self, dtype=torch.float64) -> torch.Tensor:
+        return torch.zeros(self.m(), dtype=dtype, device="cuda")
+
+    def vmult(self, dst: torch.Tensor, src: torch.Tensor) -> None:
+        """dst = A src (constrained rows: identity), CEED_bp/include/portable_laplace_operator.h:124-172."""
+        self._check_vec(dst)
+        self._check_vec(src)
+        if dst.dtype == torch.float64:
+            check(lib().mfx_op_vmult(self._h, _ptr(dst), _ptr(src), _stream()))
+        elif dst.dtype == torch.float32:
+            check(lib().mfx_op_vmult_f32(self._h, _ptr(dst), _ptr(src), _stream()))
+        else:
+            raise TypeError("vmult supports float64 and float32 vectors")
+
+    def Tvmult(self, dst, src) -> None:
+        self.vmult(dst, src)
+
+    def _check_vec(self, v: torch.Tensor):
+        if v.numel() != self.m():
+            raise ValueError(f"vector size {v.numel()} != m() = {self.m()}")
+
+    # --- set-up helpers -------------------------------------------------
+    def compute_diagonal(self) -> torch.Tensor:
+        d = self.initialize_dof_vector()
+        check(lib().mfx_op_diagonal(self._h, _ptr(d), _stream()))
+        return d
+
+    def compute_inverse_diagonal(self) -> torch.Tensor:
+        d = self.initialize_dof_vector()
+        check(lib().mfx_op_inverse_diagonal(self._h, _ptr(d), _stream()))
+        return d
+
+    def rhs(self, f_kind: int = 0) -> torch.Tensor:
+        b = self.initialize_dof_vector()
+        check(lib().mfx_op_rhs(self._h, f_kind, _ptr(b), _stream()))
+        return b
+
+    def zero_constrained(self, v: torch.Tensor) -> None:
+        check(lib().mfx_vec_zero_constrained(self._h, _ptr(v), _stream()))
+
+    def dof_coordinates(self) -> torch.Tensor:
+        c = torch.zeros(self.m() * 3, dtype=torch.float64, device="cuda")
+        check(lib().mfx_op_dof_coordinates(self._h, _ptr(c), _stream()))
+        return c.view(-1, 3)
+
+    def device_array(self, name: str, dtype, count: int) -> torch.Tensor:
+        """Copy of an operator-owned device array (map, G, B, D, cell_vertices,
+        constrained, xq, wq) as a new tensor."""
+        return _device_view(getattr(self.info, name), dtype, count).clone()
+
+
+_TYPESTR = {torch.float64: "<f8", torch.float32: "<f4", torch.int32: "<i4"}
+
+
+class _DevArray:
+    def __init__(self, ptr, typestr, count):
+        self.__cuda_array_interface__ = {"shape": (count,), "typestr": typestr,
+                                         "data": (ptr, False), "version": 2}
+
+
+def _device_view(ptr: int, dtype, count: int) -> torch.Tensor:
+    """Zero-copy tensor view of device memory owned by libmfx (uint32 arrays
+    are viewed as int32)."""
+    return torch.as_tensor(_DevArray(ptr, _TYPESTR[dtype], count), device="cuda")
+
+
+@dataclass
+class ReductionControl:
+    """deal.II ReductionControl(max_steps, tolerance, reduction)."""
+    max_steps: int = 100
+    tolerance: float = 1e-10
+    reduction: float = 1e-2
+    _last_step: int = 0
+    _initial: float = 0.0
+    _last: float = 0.0
+    seconds: float = 0.0
+
+    def last_step(self) -> int:
+        return self._last_step
+
+    def initial_value(self) -> float:
+        return self._initial
+
+    def last_value(self) -> float:
+        return self._last
+
+
+class PreconditionIdentity:
+    pass
+
+
+class DiagonalMatrix:
+    """deal.II DiagonalMatrix holding the inverse diagonal (Jacobi)."""
+
+    def __init__(self, inverse_diagonal: torch.Tensor):
+        self.vector = inverse_diagonal
+
+    def get_vector(self) -> torch.Tensor:
+        return self.vector
+
+
+class SolverCG:
+    """deal.II SolverCG<Vector>(control).solve(A, x, b, preconditioner); the
+    whole iteration runs on the GPU (mfx_cg_solve)."""
+
+    def __init__(self, control: ReductionControl):
+        self.control = control
+
+    def solve(self, A: LaplaceOperator, x: torch.Tensor, b: torch.Tensor, preconditioner) -> None:
+        A._check_vec(x)
+        A._check_vec(b)
+        if isinstance(preconditioner, DiagonalMatrix):
+            inv = _ptr(preconditioner.vector)
+        elif isinstance(preconditioner, PreconditionIdentity) or preconditioner is None:
+            inv = None
+        else:
+            raise TypeError("preconditioner must be PreconditionIdentity or DiagonalMatrix")
+        res = _lib.CgResult()
+        st = lib().mfx_cg_solve(A._h, _ptr(x), _ptr(b), inv, self.control.max_steps,
+                                self.control.tolerance, self.control.reduction, C.byref(res),
+                                _stream())
+        c = self.control
+        c._last_step, c._initial, c._last, c.seconds = (res.steps, res.initial_value,
+                                                        res.last_value, res.seconds)
+        if st == 4 or st == 6:
+            raise NoConvergence(res.steps, res.last_value)
+        check(st)
+
+
+# --- kernel-level plug point -------------------------------------------
+def bk5_apply(degree, D, G, src, dst, map_, n_cells):
+    """dst += sum_cells P^T D^T G D P src (GLL collocation), no zeroing."""
+    fn = lib().mfx_bk5_apply_f64 if src.dtype == torch.float64 else lib().mfx_bk5_apply_f32
+    check(fn(degree, _ptr(D), _ptr(G), _ptr(src), _ptr(dst), _ptr(map_), n_cells, _stream()))
+
+
+def bk3_apply(degree, nq, B, D, G, src, dst, map_, n_cells):
+    fn = lib().mfx_bk3_apply_f64 if src.dtype == torch.float64 else lib().mfx_bk3_apply_f32
+    check(fn(degree, nq, _ptr(B), _ptr(D), _ptr(G), _ptr(src), _ptr(dst), _ptr(map_), n_cells,
+             _stream()))
+
+
+def evec_apply(degree, nq, collocation, B, D, G, inp, out, n_cells):
+    if collocation:
+        check(lib().mfx_bk5_evec_f64(degree, _ptr(D), _ptr(G), _ptr(inp), _ptr(out), n_cells,
+                                     _stream()))
+    else:
+        check(lib().mfx_bk3_evec_f64(degree, nq, _ptr(B), _ptr(D), _ptr(G), _ptr(inp), _ptr(out),
+                                     n_cells, _stream()))
+
+
+def gather(map_, n_cells, nodes_per_cell, src):
+    """E-vector of src through the DoF map (invalid entries -> 0)."""
+    ev = torch.empty(n_cells * nodes_per_cell, dtype=torch.float64, device="cuda")
+    check(lib().mfx_gather_f64(_ptr(map_), n_cells, nodes_per_cell, _ptr(src), _ptr(ev), _stream()))
+    return ev
+
+
+def scatter_add(map_, n_cells, nodes_per_cell, evec, dst):
+    check(lib().mfx_scatter_add_f64(_ptr(map_), n_cells, nodes_per_cell, _ptr(evec), _ptr(dst),
+                                    _stream()))
+
+
+def random_vector(n: int, seed: int) -> torch.Tensor:
+    v = torch.empty(n, dtype=torch.float64, device="cuda")
+    check(lib().mfx_vec_random(_ptr(v), n, seed, _stream()))
+    return v

@@ -1,0 +1,231 @@
+/*
+ * mfx.h -- C-ABI of the MI355X (gfx950) matrix-free Laplacian library
+ * (benchmarks_amd/libmfx.so).
+ *
+ * Plain pointers and sizes only.  All vector arguments are DEVICE pointers
+ * (HBM-resident, like deal.II's LinearAlgebra::distributed::Vector<number,
+ * MemorySpace::Default>::get_values(), CEED_bp/include/portable_laplace_operator.h:135-136);
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ * Calls are stream-ordered and asynchronous unless stated otherwise.  One
+ * operator handle must not be used from two host threads at once.
+ *
+ * Conventions: reference cell [0,1]^3; tensor index x fastest
+ * (l = a + n (b + n c)); DoF map uint32 [n_cells][(p+1)^3] with
+ * MFX_INVALID_DOF (0xFFFFFFFF) for Dirichlet-constrained DoFs; geometry
+ * G[cell][6][q^3], components (xx, xy, xz, yy, yz, zz) of JxW * J^-1 J^-T.
+ */
+#ifndef MFX_H
+#define MFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFX_INVALID_DOF 0xFFFFFFFFu
+
+/* Status codes.  Replaces deal.II's Assert/ExcMessage and the exceptions
+ * caught in main (CEED_bp/src/bp3.cc:600-624) and Kokkos' allocation
+ * failures (bakeoff_problems_dealii/tests/scaling_bp35_kokkos_kernel/results/1_GPU.out:678). */
+typedef enum {
+  MFX_OK = 0,
+  MFX_ERR_INVALID_ARG = 1,
+  MFX_ERR_OOM = 2,
+  MFX_ERR_HIP = 3,
+  MFX_ERR_NOT_CONVERGED = 4, /* SolverControl::NoConvergence */
+  MFX_ERR_UNSUPPORTED = 5,
+  MFX_ERR_BREAKDOWN = 6 /* NaN / zero curvature inside CG */
+} mfx_status;
+
+typedef enum { MFX_QUAD_GAUSS = 0, MFX_QUAD_GLL = 1 } mfx_quadrature;
+typedef enum { MFX_GEOM_STORED = 0, MFX_GEOM_ON_THE_FLY = 1 } mfx_geometry_mode;
+
+/* Thread-local message for the last failing call on this host thread. */
+const char *mfx_last_error(void);
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mfx_version(void);
+
+/* ----------------------------------------------------------------------
+ * Kernel-level plug point.
+ * Replaces BK3::Parallel::KokkosKernel<dim,nm,nq,number>(shape_values,
+ * co_shape_gradients, G, src, dst, dof_indices, n_cells, numBlocks,
+ * threadsPerBlock) (CEED_bp/include/bk3_kokkos_kernel.h:27-38, called once
+ * per colour at CEED_bp/include/portable_laplace_operator.h:154-164) and
+ * KokkosKernel_1D_Block (bakeoff_problems_dealii/include/bk3_kokkos_kernel.h:32-434).
+ * dst += sum_cells P^T B^T D^T G D B P src over the given cells, skipping
+ * MFX_INVALID_DOF entries of `map` (gather gives 0, scatter skips).
+ * B[q*(p+1)+m]: nodal basis m at quadrature point q; D[q*nq+k]: collocation
+ * derivative on the quadrature points.  Degrees p = 1..8, nq = p+1 or p+2.
+ * ---------------------------------------------------------------------- */
+mfx_status mfx_bk3_apply_f64(int p, int nq, const double *B, const double *D,
+                             const double *G, const double *src, double *dst,
+                             const uint32_t *map, uint32_t n_cells,
+                             void *stream);
+mfx_status mfx_bk3_apply_f32(int p, int nq, const float *B, const float *D,
+                             const float *G, const float *src, float *dst,
+                             const uint32_t *map, uint32_t n_cells,
+                             void *stream);
+/* GLL collocation (BK5, B = I, nq = p+1): the L-vector form of
+ * BK5::Parallel::LaplaceOperator (CEED_BK/include/kernels/BK5/templated_cuda_kernels.cuh:11-126). */
+mfx_status mfx_bk5_apply_f64(int p, const double *D, const double *G,
+                             const double *src, double *dst,
+                             const uint32_t *map, uint32_t n_cells,
+                             void *stream);
+mfx_status mfx_bk5_apply_f32(int p, const float *D, const float *G,
+                             const float *src, float *dst, const uint32_t *map,
+                             uint32_t n_cells, void *stream);
+
+/* E-vector (no gather/scatter) element kernels: out[e] = K_e in[e],
+ * the standalone kernels of CEED_BK/include/kernels/BK{3,5}/templated_cuda_kernels.cuh
+ * (BK5 :11-126, BK3 :11-290).  in/out[e][(p+1)^3], x fastest. */
+mfx_status mfx_bk3_evec_f64(int p, int nq, const double *B, const double *D,
+                            const double *G, const double *in, double *out,
+                            uint32_t n_cells, void *stream);
+mfx_status mfx_bk5_evec_f64(int p, const double *D, const double *G,
+                            const double *in, double *out, uint32_t n_cells,
+                            void *stream);
+
+/* Gather / scatter halves of the element kernels (L-vector <-> E-vector),
+ * steps 1 and 12 of CEED_bp/include/bk3_kokkos_kernel.h:113-143, 356-382:
+ * evec[e][l] = map[e][l] == MFX_INVALID_DOF ? 0 : src[map[e][l]];
+ * dst[map[e][l]] += evec[e][l] for valid entries. */
+mfx_status mfx_gather_f64(const uint32_t *map, uint32_t n_cells, int nodes_per_cell,
+                          const double *src, double *evec, void *stream);
+mfx_status mfx_scatter_add_f64(const uint32_t *map, uint32_t n_cells, int nodes_per_cell,
+                               const double *evec, double *dst, void *stream);
+
+/* ----------------------------------------------------------------------
+ * Operator handle: Portable::LaplaceOperator<dim,fe_degree,nq,number>
+ * (CEED_bp/include/portable_laplace_operator.h:17-96) and
+ * LaplaceOperatorBK35 (bakeoff_problems_dealii/include/portable_laplace_operator_bk35.h:16-443).
+ * The handle owns the DoF map, geometry, basis tables and scratch; the
+ * caller owns every vector.
+ * ---------------------------------------------------------------------- */
+typedef struct mfx_op mfx_op;
+
+/* Structured box mesh [lo,hi] with n[d] cells per direction, optionally
+ * deformed on the vertices by x_d += deform * sin(2 pi x) sin(2 pi y)
+ * sin(2 pi z); homogeneous Dirichlet on the whole boundary.  This replaces
+ * the set-up sequence GridGenerator::subdivided_hyper_rectangle +
+ * refine_global + distribute_dofs + interpolate_boundary_values +
+ * LaplaceOperator ctor (CEED_bp/src/bp3.cc:129-182, 452-488,
+ * CEED_bp/include/portable_laplace_operator.h:97-122); mesh, map and
+ * geometry are generated on the GPU. */
+typedef struct {
+  int degree;        /* p = 1..8 */
+  int n_q_points_1d; /* nq = p+1 or p+2 */
+  int quadrature;    /* mfx_quadrature */
+  int geometry;      /* mfx_geometry_mode */
+  uint32_t n_cells[3];
+  double lo[3];
+  double hi[3];
+  double deform;
+  int with_fp32; /* also build the float copies used by mfx_op_vmult_f32 */
+} mfx_box_desc;
+
+mfx_status mfx_op_create_box(const mfx_box_desc *desc, mfx_op **out);
+
+/* General (unstructured) creation from host arrays: the data an adapter
+ * takes from deal.II's MatrixFree (dof_indices of
+ * setup_dirichlet_boundary_dofs_masks, CEED_bp/include/portable_laplace_operator.h:304-394)
+ * plus the 8 vertices of every cell (lexicographic vertex order) and the
+ * constrained DoF list (AffineConstraints). */
+typedef struct {
+  int degree;
+  int n_q_points_1d;
+  int quadrature;
+  int geometry;
+  uint32_t n_cells;
+  uint64_t n_dofs;
+  const uint32_t *map;          /* host [n_cells][(p+1)^3] */
+  const double *cell_vertices;  /* host [n_cells][8][3] */
+  const uint32_t *constrained;  /* host list of constrained DoFs */
+  uint64_t n_constrained;
+  int with_fp32;
+} mfx_op_desc;
+
+mfx_status mfx_op_create(const mfx_op_desc *desc, mfx_op **out);
+mfx_status mfx_op_destroy(mfx_op *op);
+
+/* m() / n() (CEED_bp/include/portable_laplace_operator.h:432-443). */
+uint64_t mfx_op_n_dofs(const mfx_op *op);
+uint32_t mfx_op_n_cells(const mfx_op *op);
+
+/* Device pointers owned by the handle (for kernel-level calls and tests). */
+typedef struct {
+  int degree, n_q_points_1d, quadrature, geometry;
+  uint32_t n_cells;
+  uint64_t n_dofs;
+  uint64_t n_constrained;
+  const uint32_t *map;         /* [n_cells][(p+1)^3] */
+  const double *G;             /* [n_cells][6][nq^3] (stored mode) */
+  const double *B, *D;         /* basis tables */
+  const double *cell_vertices; /* [n_cells][8][3] */
+  const uint32_t *constrained; /* [n_constrained] */
+  const double *xq, *wq;       /* 1D quadrature points/weights on [0,1] */
+} mfx_op_info;
+mfx_status mfx_op_get_info(const mfx_op *op, mfx_op_info *info);
+
+/* vmult (CEED_bp/include/portable_laplace_operator.h:124-172): dst = 0;
+ * dst += A_free src; dst_c = src_c on constrained DoFs
+ * (copy_constrained_values).  Tvmult is the same call (A is symmetric,
+ * :397-411).  src and dst must not alias. */
+mfx_status mfx_op_vmult(mfx_op *op, double *dst, const double *src,
+                        void *stream);
+/* The same in FP32 (geometry rounded to float at set-up; needs with_fp32). */
+mfx_status mfx_op_vmult_f32(mfx_op *op, float *dst, const float *src,
+                            void *stream);
+
+/* Diagonal of the assembled operator (constrained rows: 1), the quantity
+ * MatrixFreeTools::compute_diagonal fills at bp5_kokkos/benchmark.cc:231-236;
+ * and its inverse per bp5_kokkos/benchmark.cc:240-250 (1/d if d > 0, else 1). */
+mfx_status mfx_op_diagonal(mfx_op *op, double *diag, void *stream);
+mfx_status mfx_op_inverse_diagonal(mfx_op *op, double *inv_diag, void *stream);
+
+/* Right-hand side b_i = sum_q phi_i f JxW, constrained entries 0
+ * (compute_rhs, CEED_bp/src/bp3.cc:184-239).  f_kind 0: f = 1;
+ * f_kind 1: f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z). */
+mfx_status mfx_op_rhs(mfx_op *op, int f_kind, double *b, void *stream);
+
+/* Physical coordinates of every DoF, [n_dofs][3]. */
+mfx_status mfx_op_dof_coordinates(mfx_op *op, double *coords, void *stream);
+
+/* ----------------------------------------------------------------------
+ * Conjugate gradients: deal.II SolverCG<Vector>::solve(A, x, b, P) with
+ * ReductionControl(max_steps, abs_tol, rel_tol) (CEED_bp/src/bp3.cc:264-288,
+ * PreconditionIdentity; bp5_kokkos/benchmark.cc:355-368, DiagonalMatrix).
+ * inv_diag == NULL: identity preconditioner; otherwise Jacobi with the
+ * given inverse diagonal.  x holds the start vector and receives the
+ * solution.  Synchronous: returns after convergence or failure.
+ * Stops at ||r_k||_2 <= max(abs_tol, rel_tol ||r_0||_2).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int steps;            /* SolverControl::last_step() */
+  int converged;        /* 1 success, 0 failure */
+  double initial_value; /* ||r_0||_2 */
+  double last_value;    /* ||r_last||_2 */
+  double seconds;       /* wall time of the solve */
+} mfx_cg_result;
+
+mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b,
+                        const double *inv_diag, int max_steps, double abs_tol,
+                        double rel_tol, mfx_cg_result *result, void *stream);
+
+/* ----------------------------------------------------------------------
+ * Vector helpers (deal.II LA::distributed::Vector device operations).
+ * ---------------------------------------------------------------------- */
+/* Synthetic input: uniform[-1,1) from SplitMix64 with the given seed. */
+mfx_status mfx_vec_random(double *x, uint64_t n, uint64_t seed, void *stream);
+/* Set the listed entries to zero (constrained rows of a right-hand side). */
+mfx_status mfx_vec_zero_constrained(mfx_op *op, double *x, void *stream);
+/* Deterministic dot product; result written to *host_result (synchronous). */
+mfx_status mfx_vec_dot(const double *x, const double *y, uint64_t n,
+                       double *host_result, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFX_H */
