@@ -75,6 +75,12 @@ def bench_degree(p, steps, warmup, do_cg):
     torch.cuda.synchronize()
     t_kernel = k0.elapsed_time(k1) * 1e-3 / steps
     del D, G, mp, scratch
+    # known-size probes for hardware-counter scaling: the deterministic dot
+    # reads exactly 16 N bytes, vmult's zero-fill writes exactly 8 N bytes
+    from benchmarks_amd._lib import check, lib
+    dot = C.c_double()
+    check(lib().mfx_vec_dot(C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), N,
+                            C.byref(dot), C.c_void_p(stream.cuda_stream)))
     res = {"p": p, "n_cells_1d": n, "n_dofs": N, "n_cells": E, "ms_vmult": t_vmult * 1e3,
            "gdofs_vmult": N / t_vmult * 1e-9, "ms_kernel": t_kernel * 1e3,
            "bytes_alg": algorithmic_bytes(p, N, E),
@@ -95,6 +101,24 @@ def bench_degree(p, steps, warmup, do_cg):
     del op, x, y
     torch.cuda.empty_cache()
     return res
+
+
+def stream_copy_GBs(n=1 << 28, reps=10):
+    """Measured HBM ceiling: device-to-device copy of 2 GiB (read + write bytes)."""
+    import torch
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 16.0 * n / t * 1e-9
 
 
 def _cpu_worker(args):
@@ -180,6 +204,7 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline(degrees)
     import torch
     torch.cuda.synchronize()
+    stream_copy = stream_copy_GBs()
     per = [bench_degree(p, args.steps, args.warmup, not args.no_cg) for p in degrees]
     t_sum = sum(r["ms_vmult"] for r in per) * 1e-3
     n_sum = sum(r["n_dofs"] for r in per)
@@ -205,6 +230,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel": f"bk5 apply p={dom['p']}", "ms_per_launch": dom["ms_kernel"]},
         "per_degree": per,
+        "stream_copy_GBs": stream_copy,
     }
     if cpu is not None:
         out["cpu_baseline"] = cpu

@@ -57,6 +57,7 @@ _P = C.c_void_p
 _SIGS = {
     "mfx_last_error": (C.c_char_p, []),
     "mfx_version": (C.c_int, []),
+    "mfx_basis_tables": (C.c_int, [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]),
     "mfx_bk3_apply_f64": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, C.c_uint32, _P]),
     "mfx_bk3_apply_f32": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, C.c_uint32, _P]),
     "mfx_bk5_apply_f64": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_uint32, _P]),

@@ -1,0 +1,194 @@
+// bp_driver.cc -- the reference's BP3 timing driver on the mfx C++ API.
+//
+// Mirrors LaplaceProblem::run/solve of CEED_bp/src/bp3.cc:246-329, 423-534:
+// for each mesh cycle c (n_refine = c/3, rem = c%3, box [-1,-1,-1]-p2 with
+// p2_d = 2.8 and 2 subdivisions for d < rem, else 0.9 and 1; :452-474) whose
+// DoF count lies in [minsize, maxsize]: right-hand side f = 1, 10 CG solves
+// with ReductionControl(1e9, 1e-16, 1e-9) and PreconditionIdentity (best
+// time kept), then 5 x n_mv matvecs (n_mv = 200 below 1e7 DoFs, else 50),
+// and the same convergence table.
+//
+//   bp_driver <degree> [minsize] [maxsize] [nq]      (nq defaults to degree+2)
+//   bp_driver --bp5 <degree> <n> [reps]              BP5: GLL, deformed unit cube,
+//                                                    Jacobi-CG to 1e-8 from a random rhs
+//   bp_driver --verify <bk3|bk5> <nq> <nelmt>        the verification fixture of
+//       CEED_BK/src/BK3/serial_verification.cc:25-79 and
+//       sum_factorization/tests/BK5/serial_verification.cc:16-47 (G = 2, in = 3,
+//       cos() shape tables) through the GPU E-vector kernels; prints the norm
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../include/mfx.hpp"
+
+using Clock = std::chrono::steady_clock;
+using VectorD = mfx::Vector<double>;
+
+static void check_hip_sync() { mfx::check_hip(hipDeviceSynchronize()); }
+
+static double seconds_since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+static int run_bp3(unsigned degree, std::size_t minsize, std::size_t maxsize, unsigned nq) {
+  std::printf("Testing FE_Q<3>(%u), QGauss(%u)\nNo. of GPUs: 1\n", degree, nq);
+  std::printf(" cells     dofs    matvec   CG_tot_time CG_time/iters cg_its cg_reduction\n");
+  for (unsigned cycle = 0; cycle < 38; ++cycle) {
+    const unsigned n_refine = cycle / 3, remainder = cycle % 3;
+    mfx::BoxMesh mesh;
+    std::size_t projected = 1;
+    for (unsigned d = 0; d < 3; ++d) {
+      mesh.p1[d] = -1.0;
+      mesh.p2[d] = d < remainder ? 2.8 : 0.9;
+      mesh.n_cells[d] = (d < remainder ? 2u : 1u) << n_refine;
+      projected *= std::size_t(mesh.n_cells[d]) * degree + 1;
+    }
+    if (projected < minsize) continue;
+    if (projected > maxsize) {
+      std::printf("Projected size %zu higher than max size, terminating.\n", projected);
+      break;
+    }
+    mfx::LaplaceOperator<double> op(mesh, degree, nq, mfx::Quadrature::gauss);
+    VectorD x, b;
+    op.initialize_dof_vector(x);
+    op.compute_rhs(b, 0);
+    double time_cg = 1e10;
+    unsigned its = 0;
+    double reduction = 0;
+    for (int i = 0; i < 10; ++i) {
+      mfx::ReductionControl control(1000000000u, 1e-16, 1e-9);
+      mfx::SolverCG<VectorD> cg(control);
+      x = 0.;
+      check_hip_sync();
+      const auto t0 = Clock::now();
+      cg.solve(op, x, b, mfx::PreconditionIdentity());
+      time_cg = std::min(time_cg, seconds_since(t0));
+      its = control.last_step();
+      reduction = std::pow(control.last_value() / control.initial_value(), 1.0 / its);
+    }
+    double best_mv = 1e10;
+    for (int i = 0; i < 5; ++i) {
+      const unsigned n_mv = op.m() < 10000000 ? 200 : 50;
+      check_hip_sync();
+      const auto t0 = Clock::now();
+      for (unsigned k = 0; k < n_mv; ++k) op.vmult(x, b);
+      check_hip_sync();
+      best_mv = std::min(best_mv, seconds_since(t0) / n_mv);
+    }
+    std::printf("%7u %9llu %.3e %.3e %.3e %6u %.3e\n", op.n_cells(),
+                (unsigned long long)op.m(), best_mv, time_cg, time_cg / its, its, reduction);
+    std::printf("RESULT cycle=%u cells=%u dofs=%llu matvec_s=%.6e dofs_per_s=%.6e cg_its=%u "
+                "cg_s=%.6e\n",
+                cycle, op.n_cells(), (unsigned long long)op.m(), best_mv, op.m() / best_mv, its,
+                time_cg);
+    std::fflush(stdout);
+  }
+  return 0;
+}
+
+static int run_bp5(unsigned degree, unsigned n, int reps) {
+  mfx::BoxMesh mesh;
+  for (int d = 0; d < 3; ++d) mesh.n_cells[d] = n;
+  mesh.deform = 0.05;
+  mfx::LaplaceOperator<double> op(mesh, degree, degree + 1, mfx::Quadrature::gauss_lobatto);
+  VectorD b, x;
+  op.initialize_dof_vector(b);
+  op.initialize_dof_vector(x);
+  mfx::check(mfx_vec_random(b.get_values(), b.size(), 1, nullptr));
+  mfx::check(mfx_vec_zero_constrained(op.handle(), b.get_values(), nullptr));
+  mfx::DiagonalMatrix<VectorD> jacobi;
+  op.compute_inverse_diagonal(jacobi.get_vector());
+  double best = 1e10;
+  unsigned its = 0;
+  for (int r = 0; r < reps; ++r) {
+    mfx::ReductionControl control(100000, 1e-300, 1e-8);
+    mfx::SolverCG<VectorD> cg(control);
+    x = 0.;
+    check_hip_sync();
+    const auto t0 = Clock::now();
+    cg.solve(op, x, b, jacobi);
+    best = std::min(best, seconds_since(t0));
+    its = control.last_step();
+  }
+  double best_mv = 1e10;
+  for (int i = 0; i < 4; ++i) {
+    check_hip_sync();
+    const auto t0 = Clock::now();
+    for (int k = 0; k < 50; ++k) op.vmult(x, b);
+    check_hip_sync();
+    best_mv = std::min(best_mv, seconds_since(t0) / 50);
+  }
+  std::printf(" p |  q | n_element |     n_dofs |     time/it |   dofs/s/it | itCG | time/matvec\n");
+  std::printf("%2u | %2u |%10u |%11llu | %11.4e | %11.4e | %4u | %11.4e\n", degree, degree + 1,
+              op.n_cells(), (unsigned long long)op.m(), best / its, op.m() / best * its, its,
+              best_mv);
+  std::printf("RESULT bp5 p=%u cells=%u dofs=%llu matvec_s=%.6e cg_its=%u cg_s=%.6e\n", degree,
+              op.n_cells(), (unsigned long long)op.m(), best_mv, its, best);
+  return 0;
+}
+
+// E-vector kernels on the reference's verification fixture.
+static int run_verify(const std::string &kind, unsigned nq, unsigned nelmt) {
+  const bool bk5 = kind == "bk5";
+  const unsigned nm = bk5 ? nq : nq - 1;
+  const std::size_t nm3 = std::size_t(nm) * nm * nm, nq3 = std::size_t(nq) * nq * nq;
+  std::vector<double> basis(std::size_t(nq) * nm), dbasis(std::size_t(nq) * nq);
+  for (unsigned p = 0; p < nq; ++p)
+    for (unsigned i = 0; i < nm; ++i) basis[p * nm + i] = std::cos(double(p * nm + i));
+  for (unsigned i = 0; i < nq; ++i)
+    for (unsigned p = 0; p < nq; ++p) dbasis[i * nq + p] = std::cos(double(i * nq + p));
+  VectorD B(basis.size()), D(dbasis.size()), G(nelmt * 6 * nq3), in(nelmt * nm3), out(nelmt * nm3);
+  B.import_elements(basis);
+  D.import_elements(dbasis);
+  G.import_elements(std::vector<double>(G.size(), 2.0));
+  in.import_elements(std::vector<double>(in.size(), 3.0));
+  if (bk5)
+    mfx::check(mfx_bk5_evec_f64(nq - 1, D.get_values(), G.get_values(), in.get_values(),
+                                out.get_values(), nelmt, nullptr));
+  else
+    mfx::check(mfx_bk3_evec_f64(nm - 1, nq, B.get_values(), D.get_values(), G.get_values(),
+                                in.get_values(), out.get_values(), nelmt, nullptr));
+  check_hip_sync();
+  std::printf("RESULT verify %s nq=%u nelmt=%u norm=%.17g\n", kind.c_str(), nq, nelmt, out.l2_norm());
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  try {
+    if (argc > 1 && std::string(argv[1]) == "--verify") {
+      if (argc < 5) {
+        std::printf("Usage: bp_driver --verify bk3|bk5 nq nelmt\n");
+        return 1;
+      }
+      return run_verify(argv[2], std::atoi(argv[3]), std::atoi(argv[4]));
+    }
+    if (argc > 1 && std::string(argv[1]) == "--bp5") {
+      if (argc < 4) {
+        std::printf("Usage: bp_driver --bp5 degree n [reps]\n");
+        return 1;
+      }
+      return run_bp5(std::atoi(argv[2]), std::atoi(argv[3]), argc > 4 ? std::atoi(argv[4]) : 3);
+    }
+    if (argc == 1) {
+      std::printf("Expected at least one argument.\nUsage:\n./bp_driver degree minsize maxsize [nq]\n");
+      return 1;
+    }
+    const unsigned degree = std::atoi(argv[1]);
+    const std::size_t minsize = argc > 2 ? std::atoll(argv[2]) : 1;
+    const std::size_t maxsize = argc > 3 ? std::atoll(argv[3]) : std::size_t(-1);
+    const unsigned nq = argc > 4 ? std::atoi(argv[4]) : degree + 2;
+    return run_bp3(degree, minsize, maxsize, nq);
+  } catch (std::exception &exc) {
+    std::fprintf(stderr, "\n----------------------------------------------------\n"
+                         "Exception on processing: \n%s\nAborting!\n"
+                         "----------------------------------------------------\n",
+                 exc.what());
+    return 1;
+  }
+}

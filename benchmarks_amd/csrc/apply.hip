@@ -16,10 +16,13 @@
 //   * the map and G are read with the column index fastest, so every load
 //     instruction of a wavefront touches contiguous 8-byte runs (G layout
 //     [cell][6][q^3], x fastest);
-//   * scatter: hardware FP64/FP32 atomic add in L2 (global_atomic_add_f64);
-//   * workgroup b runs on XCD b % 8 (round-robin dispatch); the batch index is
-//     remapped so each XCD walks one contiguous eighth of the cells, keeping
-//     the src/dst lines shared by neighbouring cells inside one XCD's L2.
+//   * collocation kernel: the 6 q-columns of G a thread needs are loaded into
+//     registers before the first barrier, so the geometry stream is in flight
+//     while the dependent map -> src gather completes;
+//   * scatter: hardware FP64/FP32 atomic add in L2 (global_atomic_add_f64).
+// Cells per workgroup and the prefetch switch were chosen per nq by timing
+// the p = 2/4/6/8 BP5 operators at ~32M DoFs (a remap of workgroups onto
+// XCD-contiguous cell ranges was measured too and did not pay).
 #include <type_traits>
 
 #include "mfx_internal.hpp"
@@ -27,15 +30,19 @@
 namespace mfx {
 namespace {
 
+// Cells per workgroup (EPB) and G prefetch for the collocation kernel;
+// the interpolation kernel keeps EPB = floor(256 / nq^2).
 template <int NQ>
 struct Tile {
   static constexpr int EPB = (256 / (NQ * NQ)) > 0 ? 256 / (NQ * NQ) : 1;
   static constexpr int THREADS = EPB * NQ * NQ;
 };
-
-__device__ __forceinline__ uint32_t logical_batch(uint32_t b, uint32_t per_xcd) {
-  return (b & 7u) * per_xcd + (b >> 3);
-}
+template <int NQ>
+struct CollocTile {
+  static constexpr int EPB = NQ <= 6 ? 128 / (NQ * NQ) : (NQ == 7 ? 1 : (NQ == 8 ? 2 : 3));
+  static constexpr int THREADS = EPB * NQ * NQ;
+  static constexpr bool PREFETCH = NQ <= 7;
+};
 
 __device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
@@ -61,33 +68,39 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 // GLL collocation (BK5): NM == NQ, B = I.
 // ---------------------------------------------------------------------------
 template <typename T, int NQ, bool GATHER, bool DOT>
-__global__ void __launch_bounds__(Tile<NQ>::THREADS)
+__global__ void __launch_bounds__(CollocTile<NQ>::THREADS)
 k_apply_colloc(const T *__restrict__ gD, const T *__restrict__ G,
                const T *__restrict__ src, T *__restrict__ dst,
                const uint32_t *__restrict__ map, uint32_t n_cells,
-               uint32_t n_batches, uint32_t per_xcd, double *__restrict__ partials,
-               const int *__restrict__ skip) {
-  constexpr int EPB = Tile<NQ>::EPB, NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
+               double *__restrict__ partials, const int *__restrict__ skip) {
+  constexpr int EPB = CollocTile<NQ>::EPB, THREADS = CollocTile<NQ>::THREADS;
+  constexpr bool PREF = CollocTile<NQ>::PREFETCH;
+  constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
   __shared__ T sD[NQ * NQ];
   __shared__ T su[EPB][NQ3];
   __shared__ T sfx[EPB][NQ3];
   __shared__ T sfy[EPB][NQ3];
-  __shared__ double red[(Tile<NQ>::THREADS + 63) / 64];
+  __shared__ double red[(THREADS + 63) / 64];
 
   if (skip && *skip) return;  // converged CG: nothing left to do
-  const uint32_t batch = logical_batch(blockIdx.x, per_xcd);
-  if (batch >= n_batches) {
-    if (DOT && threadIdx.x == 0) partials[blockIdx.x] = 0.0;
-    return;
-  }
   const int tid = threadIdx.x;
   const int el = tid / NQ2;
   const int col = tid - el * NQ2;
   const int qx = col % NQ, qy = col / NQ;
-  const uint32_t e = batch * EPB + el;
+  const uint32_t e = blockIdx.x * EPB + el;
   const bool active = e < n_cells;
 
-  for (int i = tid; i < NQ * NQ; i += Tile<NQ>::THREADS) sD[i] = gD[i];
+  for (int i = tid; i < NQ * NQ; i += THREADS) sD[i] = gD[i];
+
+  // geometry columns first: independent of the gather, so in flight early
+  const T *Ge = G + (size_t)(active ? e : 0) * 6 * NQ3 + col;
+  T g[PREF ? 6 : 1][PREF ? NQ : 1];
+  if (PREF && active) {
+#pragma unroll
+    for (int z = 0; z < NQ; ++z)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) g[c][z] = __builtin_nontemporal_load(Ge + c * NQ3 + z * NQ2);
+  }
 
   uint32_t idx[NQ];
   T u[NQ];
@@ -110,15 +123,20 @@ k_apply_colloc(const T *__restrict__ gD, const T *__restrict__ G,
 
   T fz[NQ];
   if (active) {
-    const T *Ge = G + (size_t)e * 6 * NQ3 + col;
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
-      const T gxx = __builtin_nontemporal_load(Ge + 0 * NQ3 + z * NQ2);
-      const T gxy = __builtin_nontemporal_load(Ge + 1 * NQ3 + z * NQ2);
-      const T gxz = __builtin_nontemporal_load(Ge + 2 * NQ3 + z * NQ2);
-      const T gyy = __builtin_nontemporal_load(Ge + 3 * NQ3 + z * NQ2);
-      const T gyz = __builtin_nontemporal_load(Ge + 4 * NQ3 + z * NQ2);
-      const T gzz = __builtin_nontemporal_load(Ge + 5 * NQ3 + z * NQ2);
+      T gxx, gxy, gxz, gyy, gyz, gzz;
+      if constexpr (PREF) {
+        gxx = g[0][z]; gxy = g[1][z]; gxz = g[2][z];
+        gyy = g[3][z]; gyz = g[4][z]; gzz = g[5][z];
+      } else {
+        gxx = __builtin_nontemporal_load(Ge + 0 * NQ3 + z * NQ2);
+        gxy = __builtin_nontemporal_load(Ge + 1 * NQ3 + z * NQ2);
+        gxz = __builtin_nontemporal_load(Ge + 2 * NQ3 + z * NQ2);
+        gyy = __builtin_nontemporal_load(Ge + 3 * NQ3 + z * NQ2);
+        gyz = __builtin_nontemporal_load(Ge + 4 * NQ3 + z * NQ2);
+        gzz = __builtin_nontemporal_load(Ge + 5 * NQ3 + z * NQ2);
+      }
       T dx = 0, dy = 0, dz = 0;
 #pragma unroll
       for (int a = 0; a < NQ; ++a) {
@@ -153,7 +171,7 @@ k_apply_colloc(const T *__restrict__ gD, const T *__restrict__ G,
     }
   }
   if (DOT) {
-    const double s = block_sum<Tile<NQ>::THREADS>(dot, red);
+    const double s = block_sum<THREADS>(dot, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
   }
 }
@@ -167,8 +185,8 @@ __global__ void __launch_bounds__(Tile<NQ>::THREADS)
 k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
                const T *__restrict__ G, const T *__restrict__ src,
                T *__restrict__ dst, const uint32_t *__restrict__ map,
-               uint32_t n_cells, uint32_t n_batches, uint32_t per_xcd,
-               double *__restrict__ partials, const int *__restrict__ skip) {
+               uint32_t n_cells, double *__restrict__ partials,
+               const int *__restrict__ skip) {
   constexpr int EPB = Tile<NQ>::EPB, NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
   constexpr int NM2 = NM * NM, NM3 = NM2 * NM;
   __shared__ T sB[NQ * NM];
@@ -179,16 +197,11 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   __shared__ double red[(Tile<NQ>::THREADS + 63) / 64];
 
   if (skip && *skip) return;  // converged CG: nothing left to do
-  const uint32_t batch = logical_batch(blockIdx.x, per_xcd);
-  if (batch >= n_batches) {
-    if (DOT && threadIdx.x == 0) partials[blockIdx.x] = 0.0;
-    return;
-  }
   const int tid = threadIdx.x;
   const int el = tid / NQ2;
   const int col = tid - el * NQ2;
   const int qx = col % NQ, qy = col / NQ;
-  const uint32_t e = batch * EPB + el;
+  const uint32_t e = blockIdx.x * EPB + el;
   const bool active = e < n_cells;
   const bool node_col = active && qx < NM && qy < NM;
 
@@ -333,10 +346,10 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   }
 }
 
-template <int NQ>
+template <int NQ, bool COLLOC>
 uint32_t n_blocks_for(uint32_t n_cells) {
-  const uint32_t nb = (n_cells + Tile<NQ>::EPB - 1) / Tile<NQ>::EPB;
-  return 8u * ((nb + 7u) / 8u);
+  constexpr int EPB = COLLOC ? CollocTile<NQ>::EPB : Tile<NQ>::EPB;
+  return (n_cells + EPB - 1) / EPB;
 }
 
 template <typename T, int NM, int NQ, bool COLLOC, bool GATHER>
@@ -344,27 +357,27 @@ void launch_t(const T *B, const T *D, const T *G, const T *src, T *dst,
               const uint32_t *map, uint32_t n_cells, hipStream_t s,
               double *partials, const int *skip) {
   if (n_cells == 0) return;
-  const uint32_t nb = (n_cells + Tile<NQ>::EPB - 1) / Tile<NQ>::EPB;
-  const uint32_t per = (nb + 7u) / 8u;
-  const dim3 grid(8u * per), block(Tile<NQ>::THREADS);
+  const dim3 grid(n_blocks_for<NQ, COLLOC>(n_cells));
   if constexpr (COLLOC) {
     static_assert(NM == NQ, "collocation needs nm == nq");
+    const dim3 block(CollocTile<NQ>::THREADS);
     if (partials) {
       if constexpr (std::is_same<T, double>::value && GATHER)
         hipLaunchKernelGGL((k_apply_colloc<T, NQ, GATHER, true>), grid, block, 0, s,
-                           D, G, src, dst, map, n_cells, nb, per, partials, skip);
+                           D, G, src, dst, map, n_cells, partials, skip);
     } else {
       hipLaunchKernelGGL((k_apply_colloc<T, NQ, GATHER, false>), grid, block, 0, s,
-                         D, G, src, dst, map, n_cells, nb, per, nullptr, nullptr);
+                         D, G, src, dst, map, n_cells, nullptr, nullptr);
     }
   } else {
+    const dim3 block(Tile<NQ>::THREADS);
     if (partials) {
       if constexpr (std::is_same<T, double>::value && GATHER)
         hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, true>), grid, block, 0, s,
-                           B, D, G, src, dst, map, n_cells, nb, per, partials, skip);
+                           B, D, G, src, dst, map, n_cells, partials, skip);
     } else {
       hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, false>), grid, block, 0, s,
-                         B, D, G, src, dst, map, n_cells, nb, per, nullptr, nullptr);
+                         B, D, G, src, dst, map, n_cells, nullptr, nullptr);
     }
   }
   MFX_HIP(hipGetLastError());
@@ -428,20 +441,15 @@ template void launch_evec<double>(int, int, bool, const double *, const double *
                                   hipStream_t);
 
 uint32_t apply_n_blocks(int p, int nq, bool colloc, uint32_t n_cells) {
-  (void)colloc;
   (void)p;
+#define MFX_NB(Q)                                                                    \
+  case Q:                                                                           \
+    return colloc ? n_blocks_for<Q, true>(n_cells) : n_blocks_for<Q, false>(n_cells);
   switch (nq) {
-    case 2: return n_blocks_for<2>(n_cells);
-    case 3: return n_blocks_for<3>(n_cells);
-    case 4: return n_blocks_for<4>(n_cells);
-    case 5: return n_blocks_for<5>(n_cells);
-    case 6: return n_blocks_for<6>(n_cells);
-    case 7: return n_blocks_for<7>(n_cells);
-    case 8: return n_blocks_for<8>(n_cells);
-    case 9: return n_blocks_for<9>(n_cells);
-    case 10: return n_blocks_for<10>(n_cells);
+    MFX_NB(2) MFX_NB(3) MFX_NB(4) MFX_NB(5) MFX_NB(6) MFX_NB(7) MFX_NB(8) MFX_NB(9) MFX_NB(10)
     default: return 0;
   }
+#undef MFX_NB
 }
 
 }  // namespace mfx

@@ -128,6 +128,16 @@ static void common_init(mfx_op *op, int p, int nq, int quad, int geometry) {
 extern "C" {
 
 const char *mfx_last_error(void) { return g_last_error.c_str(); }
+
+mfx_status mfx_basis_tables(int p, int nq, int quadrature, double *B, double *D, double *xq,
+                            double *wq) {
+  return guard([&] {
+    MFX_REQUIRE(B && D && xq && wq, "null pointer argument");
+    MFX_REQUIRE(quadrature == MFX_QUAD_GAUSS || quadrature == MFX_QUAD_GLL, "unknown quadrature");
+    std::vector<double> sup(p + 1 > 0 ? p + 1 : 1);
+    basis_tables(p, nq, quadrature, B, D, xq, wq, sup.data());
+  });
+}
 int mfx_version(void) { return 100; }
 
 // ---------------------------------------------------------------- kernels

@@ -1,0 +1,75 @@
+"""Summaries of rocprofv3 CSV output as short markdown (for profiles/).
+
+  python -m benchmarks_amd.rocprof_summary stats <kernel_stats.csv> [title]
+      per-kernel calls / average / share from `rocprofv3 --kernel-trace --stats`
+  python -m benchmarks_amd.rocprof_summary traffic <fetch counter_collection.csv>
+      <write counter_collection.csv> <n_dofs> [title]
+      HBM bytes per dispatch from two `rocprofv3 --pmc FETCH_SIZE` /
+      `--pmc WRITE_SIZE` runs of `bench.py --degrees <p>`, each counter scaled
+      by what it reads for a kernel of known traffic in the same run: the
+      deterministic dot (k_dot_partials, reads 16 N bytes) for FETCH_SIZE and
+      the zero-fill (k_fill, writes 8 N bytes) for WRITE_SIZE.
+"""
+import csv
+import re
+import sys
+from collections import defaultdict
+
+
+def short(name: str) -> str:
+    name = re.sub(r"\(.*$", "", name)
+    name = name.replace("mfx::(anonymous namespace)::", "").replace("void ", "")
+    return name[:90]
+
+
+def stats(path, title="kernel trace"):
+    rows = list(csv.DictReader(open(path)))
+    out = [f"### {title}", "", "| kernel | calls | avg (us) | total (ms) | share |",
+           "|---|---:|---:|---:|---:|"]
+    for r in rows[:14]:
+        out.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.1f}% |")
+    return "\n".join(out)
+
+
+def _per_kernel(path, counter):
+    acc = defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r.get("Counter_Name", "").startswith(counter):
+            acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def traffic(fetch_csv, write_csv, n_dofs, title="HBM traffic"):
+    n_dofs = int(n_dofs)
+    fetch = _per_kernel(fetch_csv, "FETCH_SIZE")
+    write = _per_kernel(write_csv, "WRITE_SIZE")
+    f_probe = next(v for k, v in fetch.items() if "k_dot_partials" in k)
+    w_probe = next(v for k, v in write.items() if "k_fill(" in k)
+    f_scale = 16.0 * n_dofs / (f_probe * 1024.0)
+    w_scale = 8.0 * n_dofs / (w_probe * 1024.0)
+    out = [f"### {title}", "",
+           f"FETCH_SIZE scale {f_scale:.4f} (k_dot_partials reads 16 N = {16 * n_dofs} B; "
+           f"counter {f_probe:.0f} KiB); WRITE_SIZE scale {w_scale:.4f} (k_fill writes "
+           f"8 N = {8 * n_dofs} B; counter {w_probe:.0f} KiB).", "",
+           "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes / dispatch (scaled) |",
+           "|---|---:|---:|---:|"]
+    for k in sorted(set(fetch) | set(write), key=lambda k: -fetch.get(k, 0)):
+        f, w = fetch.get(k, 0.0), write.get(k, 0.0)
+        tot = f * 1024 * f_scale + w * 1024 * w_scale
+        out.append(f"| `{short(k)}` | {f:.0f} | {w:.0f} | {tot:.4e} |")
+        if "k_apply" in k:
+            out.append("")
+            out.append(f"APPLY_TRAFFIC_BYTES {short(k)} {tot:.6e}")
+            out.append("")
+    return "\n".join(out)
+
+
+if __name__ == "__main__":
+    mode = sys.argv[1]
+    if mode == "stats":
+        print(stats(sys.argv[2], *(sys.argv[3:4] or ["kernel trace"])))
+    elif mode == "traffic":
+        print(traffic(sys.argv[2], sys.argv[3], sys.argv[4], *(sys.argv[5:6] or ["HBM traffic"])))
+    else:
+        raise SystemExit(__doc__)

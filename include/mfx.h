@@ -42,6 +42,15 @@ typedef enum {
 typedef enum { MFX_QUAD_GAUSS = 0, MFX_QUAD_GLL = 1 } mfx_quadrature;
 typedef enum { MFX_GEOM_STORED = 0, MFX_GEOM_ON_THE_FLY = 1 } mfx_geometry_mode;
 
+/* Host-only: 1D tables on [0,1] for degree p and nq quadrature points
+ * (quadrature = mfx_quadrature): B[q*(p+1)+m] (FE_Q basis on Gauss-Lobatto
+ * support points at the quadrature points), D[q*nq+k] (collocation
+ * derivative), quadrature points xq[nq] and weights wq[nq].  The shape data
+ * deal.II's MatrixFree hands the reference kernel (shape_values,
+ * co_shape_gradients: CEED_bp/include/portable_laplace_operator.h:147). */
+mfx_status mfx_basis_tables(int p, int nq, int quadrature, double *B, double *D,
+                            double *xq, double *wq);
+
 /* Thread-local message for the last failing call on this host thread. */
 const char *mfx_last_error(void);
 

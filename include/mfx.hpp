@@ -1,0 +1,275 @@
+// mfx.hpp -- C++ host API over the C-ABI of include/mfx.h.
+//
+// Mirrors the pieces of deal.II the reference drivers use on this path, with
+// the same member names and meaning:
+//   Portable::LaplaceOperator<dim,fe_degree,nq,number>
+//       (CEED_bp/include/portable_laplace_operator.h:17-96):
+//       vmult, Tvmult, m, n, initialize_dof_vector;
+//   LinearAlgebra::distributed::Vector<number, MemorySpace::Default>
+//       (device vector: size, get_values, operator=(0), l2_norm, import);
+//   SolverCG, ReductionControl, PreconditionIdentity, DiagonalMatrix,
+//   SolverControl::NoConvergence (CEED_bp/src/bp3.cc:264-288,
+//   bp5_kokkos/benchmark.cc:336-378).
+// Errors become exceptions (mfx::Error / NoConvergence), as deal.II's do.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mfx.h"
+
+namespace mfx {
+
+class Error : public std::runtime_error {
+ public:
+  Error(mfx_status s, const std::string &msg) : std::runtime_error(msg), status(s) {}
+  mfx_status status;
+};
+
+inline void check(mfx_status s) {
+  if (s != MFX_OK) throw Error(s, mfx_last_error());
+}
+inline void check_hip(hipError_t e) {
+  if (e != hipSuccess) throw Error(MFX_ERR_HIP, hipGetErrorString(e));
+}
+
+// Device vector of owned size n (no ghosts on one GPU).
+template <typename Number>
+class Vector {
+ public:
+  Vector() = default;
+  explicit Vector(std::size_t n) { reinit(n); }
+  Vector(const Vector &o) { *this = o; }
+  Vector(Vector &&o) noexcept { swap(o); }
+  ~Vector() {
+    if (values_) (void)hipFree(values_);
+  }
+  Vector &operator=(const Vector &o) {
+    if (this != &o) {
+      reinit(o.size_, true);
+      if (size_) check_hip(hipMemcpy(values_, o.values_, bytes(), hipMemcpyDeviceToDevice));
+    }
+    return *this;
+  }
+  Vector &operator=(Vector &&o) noexcept {
+    swap(o);
+    return *this;
+  }
+  void swap(Vector &o) noexcept {
+    std::swap(values_, o.values_);
+    std::swap(size_, o.size_);
+  }
+  void reinit(std::size_t n, bool omit_zeroing = false) {
+    if (n != size_) {
+      if (values_) check_hip(hipFree(values_));
+      values_ = nullptr;
+      size_ = n;
+      if (n) check_hip(hipMalloc(reinterpret_cast<void **>(&values_), bytes()));
+    }
+    if (!omit_zeroing && n) check_hip(hipMemset(values_, 0, bytes()));
+  }
+  // deal.II's `v = 0.`
+  Vector &operator=(Number s) {
+    if (s != Number(0)) throw Error(MFX_ERR_INVALID_ARG, "only v = 0 is supported");
+    if (size_) check_hip(hipMemset(values_, 0, bytes()));
+    return *this;
+  }
+  std::size_t size() const { return size_; }
+  std::size_t locally_owned_size() const { return size_; }
+  Number *get_values() { return values_; }
+  const Number *get_values() const { return values_; }
+  void import_elements(const std::vector<Number> &host) {
+    if (host.size() != size_) throw Error(MFX_ERR_INVALID_ARG, "import: size mismatch");
+    check_hip(hipMemcpy(values_, host.data(), bytes(), hipMemcpyHostToDevice));
+  }
+  std::vector<Number> to_host() const {
+    std::vector<Number> h(size_);
+    if (size_) check_hip(hipMemcpy(h.data(), values_, bytes(), hipMemcpyDeviceToHost));
+    return h;
+  }
+  double operator*(const Vector &o) const {
+    static_assert(sizeof(Number) == 8, "dot products on FP64 vectors");
+    double r = 0.0;
+    check(mfx_vec_dot(values_, o.values_, size_, &r, nullptr));
+    return r;
+  }
+  double l2_norm() const;
+
+ private:
+  std::size_t bytes() const { return size_ * sizeof(Number); }
+  Number *values_ = nullptr;
+  std::size_t size_ = 0;
+};
+
+template <typename Number>
+double Vector<Number>::l2_norm() const {
+  double r = (*this) * (*this);
+  return r > 0 ? __builtin_sqrt(r) : 0.0;
+}
+
+// Structured box mesh of the reference drivers
+// (GridGenerator::subdivided_hyper_rectangle + refine_global,
+// CEED_bp/src/bp3.cc:452-488), optionally deformed on the vertices.
+struct BoxMesh {
+  unsigned n_cells[3] = {1, 1, 1};
+  double p1[3] = {0, 0, 0};
+  double p2[3] = {1, 1, 1};
+  double deform = 0.0;
+};
+
+enum class Quadrature { gauss = MFX_QUAD_GAUSS, gauss_lobatto = MFX_QUAD_GLL };
+
+template <typename Number = double>
+class LaplaceOperator {
+  static_assert(sizeof(Number) == 8 || sizeof(Number) == 4, "double or float");
+
+ public:
+  using VectorType = Vector<Number>;
+
+  LaplaceOperator(const BoxMesh &mesh, unsigned fe_degree, unsigned n_q_points_1d,
+                  Quadrature quad = Quadrature::gauss, bool with_fp32 = false) {
+    mfx_box_desc d{};
+    d.degree = static_cast<int>(fe_degree);
+    d.n_q_points_1d = static_cast<int>(n_q_points_1d);
+    d.quadrature = static_cast<int>(quad);
+    d.geometry = MFX_GEOM_STORED;
+    for (int k = 0; k < 3; ++k) {
+      d.n_cells[k] = mesh.n_cells[k];
+      d.lo[k] = mesh.p1[k];
+      d.hi[k] = mesh.p2[k];
+    }
+    d.deform = mesh.deform;
+    d.with_fp32 = (with_fp32 || sizeof(Number) == 4) ? 1 : 0;
+    check(mfx_op_create_box(&d, &op_));
+  }
+  LaplaceOperator(const LaplaceOperator &) = delete;
+  LaplaceOperator &operator=(const LaplaceOperator &) = delete;
+  ~LaplaceOperator() {
+    if (op_) mfx_op_destroy(op_);
+  }
+
+  // dst = A src; constrained rows are the identity (copy_constrained_values).
+  void vmult(VectorType &dst, const VectorType &src) const {
+    if constexpr (sizeof(Number) == 8)
+      check(mfx_op_vmult(op_, dst.get_values(), src.get_values(), nullptr));
+    else
+      check(mfx_op_vmult_f32(op_, dst.get_values(), src.get_values(), nullptr));
+  }
+  void Tvmult(VectorType &dst, const VectorType &src) const { vmult(dst, src); }
+  std::uint64_t m() const { return mfx_op_n_dofs(op_); }
+  std::uint64_t n() const { return m(); }
+  std::uint32_t n_cells() const { return mfx_op_n_cells(op_); }
+  void initialize_dof_vector(VectorType &v) const { v.reinit(m()); }
+
+  // inverse diagonal for Jacobi (MatrixFreeTools::compute_diagonal + the
+  // inversion of bp5_kokkos/benchmark.cc:231-250)
+  void compute_inverse_diagonal(Vector<double> &inv) const {
+    inv.reinit(m(), true);
+    check(mfx_op_inverse_diagonal(op_, inv.get_values(), nullptr));
+  }
+  // b_i = (phi_i, f); f_kind 0: f = 1, 1: f = 3 pi^2 prod sin(pi x_d)
+  void compute_rhs(Vector<double> &b, int f_kind) const {
+    b.reinit(m(), true);
+    check(mfx_op_rhs(op_, f_kind, b.get_values(), nullptr));
+  }
+  mfx_op *handle() const { return op_; }
+
+ private:
+  mfx_op *op_ = nullptr;
+};
+
+class SolverControl {
+ public:
+  struct NoConvergence : public std::runtime_error {
+    NoConvergence(unsigned step, double value)
+        : std::runtime_error("Iterative method reported convergence failure in step " +
+                             std::to_string(step)),
+          last_step(step), last_residual(value) {}
+    unsigned last_step;
+    double last_residual;
+  };
+  SolverControl(unsigned max_steps, double tolerance)
+      : max_steps_(max_steps), tolerance_(tolerance) {}
+  virtual ~SolverControl() = default;
+  unsigned max_steps() const { return max_steps_; }
+  double tolerance() const { return tolerance_; }
+  virtual double reduction() const { return 0.0; }
+  unsigned last_step() const { return last_step_; }
+  double initial_value() const { return initial_; }
+  double last_value() const { return last_; }
+  double seconds() const { return seconds_; }
+  void record(const mfx_cg_result &r) {
+    last_step_ = static_cast<unsigned>(r.steps);
+    initial_ = r.initial_value;
+    last_ = r.last_value;
+    seconds_ = r.seconds;
+  }
+
+ private:
+  unsigned max_steps_;
+  double tolerance_;
+  unsigned last_step_ = 0;
+  double initial_ = 0, last_ = 0, seconds_ = 0;
+};
+
+// Stops at ||r_k|| <= max(tolerance, reduction * ||r_0||).
+class ReductionControl : public SolverControl {
+ public:
+  ReductionControl(unsigned max_steps, double tolerance, double reduction)
+      : SolverControl(max_steps, tolerance), reduction_(reduction) {}
+  double reduction() const override { return reduction_; }
+
+ private:
+  double reduction_;
+};
+
+struct PreconditionIdentity {};
+
+template <typename VectorType>
+class DiagonalMatrix {
+ public:
+  VectorType &get_vector() { return v_; }
+  const VectorType &get_vector() const { return v_; }
+
+ private:
+  VectorType v_;
+};
+
+template <typename VectorType>
+class SolverCG {
+ public:
+  explicit SolverCG(SolverControl &control) : control_(control) {}
+
+  template <typename MatrixType>
+  void solve(const MatrixType &A, VectorType &x, const VectorType &b,
+             const PreconditionIdentity &) {
+    run(A, x, b, nullptr);
+  }
+  template <typename MatrixType>
+  void solve(const MatrixType &A, VectorType &x, const VectorType &b,
+             const DiagonalMatrix<VectorType> &P) {
+    run(A, x, b, P.get_vector().get_values());
+  }
+
+ private:
+  template <typename MatrixType>
+  void run(const MatrixType &A, VectorType &x, const VectorType &b, const double *inv) {
+    mfx_cg_result r{};
+    const mfx_status s =
+        mfx_cg_solve(A.handle(), x.get_values(), b.get_values(), inv,
+                     static_cast<int>(control_.max_steps()), control_.tolerance(),
+                     control_.reduction(), &r, nullptr);
+    control_.record(r);
+    if (s == MFX_ERR_NOT_CONVERGED || s == MFX_ERR_BREAKDOWN)
+      throw SolverControl::NoConvergence(control_.last_step(), control_.last_value());
+    check(s);
+  }
+  SolverControl &control_;
+};
+
+}  // namespace mfx

@@ -1,0 +1,51 @@
+"""The C++ host API (include/mfx.hpp) through the reference-style driver
+benchmarks_amd/apps/bp_driver, read like the reference's own drivers: CG
+iteration counts of CEED_bp/src/bp3.cc's mesh cycles, and the verification
+fixture norms of the reference's serial_verification programs."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "benchmarks_amd", "apps", "bp_driver")
+
+# CEED_bp/results/1xGH200_P4.txt:635-639
+KAT1 = {256: 92, 512: 99, 1024: 153, 2048: 170, 4096: 180}
+
+
+def run(*args, timeout=120):
+    assert os.path.exists(DRIVER), "bp_driver not built"
+    print("run", DRIVER, *args, flush=True)
+    out = subprocess.run([DRIVER, *map(str, args)], capture_output=True, text=True,
+                         timeout=timeout)
+    assert out.returncode == 0, out.stderr + out.stdout
+    return out.stdout
+
+
+def test_driver_bp3_cycles(gpu):
+    text = run(4, 10000, 300000)
+    rows = re.findall(r"RESULT cycle=(\d+) cells=(\d+) dofs=(\d+) .* cg_its=(\d+)", text)
+    seen = {int(c): int(its) for _, c, _, its in rows}
+    assert set(KAT1) <= set(seen), text
+    for cells, its in KAT1.items():
+        assert seen[cells] == its, (cells, seen[cells], its)
+
+
+@pytest.mark.parametrize("kind,nq", [("bk3", q) for q in range(3, 8)] +
+                         [("bk5", q) for q in range(2, 10)])
+def test_driver_verification_fixture(gpu, golden, kind, nq):
+    text = run("--verify", kind, nq, 64)
+    norm = float(re.search(r"norm=(\S+)", text).group(1))
+    ref = float(golden[f"kat3_{kind}_nq{nq}_norm"])
+    assert abs(norm - ref) <= 1e-12 * abs(ref), (norm, ref)
+
+
+def test_driver_bp5_runs(gpu):
+    text = run("--bp5", 4, 8, 1)
+    m = re.search(r"RESULT bp5 p=4 cells=512 dofs=35937 .* cg_its=(\d+)", text)
+    assert m and int(m.group(1)) > 10
