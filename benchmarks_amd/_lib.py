@@ -45,7 +45,8 @@ class OpInfo(C.Structure):
                 ("geometry", C.c_int), ("n_cells", C.c_uint32), ("n_dofs", C.c_uint64),
                 ("n_constrained", C.c_uint64), ("map", C.c_void_p), ("G", C.c_void_p),
                 ("B", C.c_void_p), ("D", C.c_void_p), ("cell_vertices", C.c_void_p),
-                ("constrained", C.c_void_p), ("xq", C.c_void_p), ("wq", C.c_void_p)]
+                ("constrained", C.c_void_p), ("xq", C.c_void_p), ("wq", C.c_void_p),
+                ("n_patches", C.c_uint32), ("patch_cells", C.c_int * 3), ("patch_map", C.c_void_p)]
 
 
 class CgResult(C.Structure):

@@ -8,6 +8,7 @@
 
 #include "cg_state.hpp"
 #include "mfx_internal.hpp"
+#include "patch.hpp"
 
 namespace mfx {
 
@@ -75,13 +76,61 @@ struct mfx_op {
   uint64_t n_cons = 0;
   DBuf<uint32_t> map, cons;
   DBuf<double> G, B, D, xq, wq, support, cellX;
-  DBuf<float> G32, B32, D32;
+  DBuf<float> G32, B32, D32, xq32, wq32;
+  // patch-of-cells kernels (box meshes, GLL collocation, on-the-fly geometry)
+  bool patch = false;
+  PatchLayout lay{};
+  PatchShape shape{};
+  DBuf<uint32_t> pmap;
   // CG scratch, allocated on first use
   DBuf<double> r, pdir, ap, part;
   DBuf<CgState> st;
   int *h_done = nullptr;  // pinned
   ~mfx_op() {
     if (h_done) (void)hipHostFree(h_done);
+  }
+
+  PatchOp<double> patch_op64() const {
+    PatchOp<double> o;
+    o.p = p;
+    o.shape = shape;
+    o.lay = lay;
+    o.D = D.ptr;
+    o.w = wq.ptr;
+    o.xq = xq.ptr;
+    o.cellX = cellX.ptr;
+    o.pmap = pmap.ptr;
+    return o;
+  }
+  PatchOp<float> patch_op32() const {
+    PatchOp<float> o;
+    o.p = p;
+    o.shape = shape;
+    o.lay = lay;
+    o.D = D32.ptr;
+    o.w = wq32.ptr;
+    o.xq = xq32.ptr;
+    o.cellX = cellX.ptr;
+    o.pmap = pmap.ptr;
+    return o;
+  }
+  uint32_t apply_blocks() const {
+    return patch ? patch_n_blocks(lay) : apply_n_blocks(p, nq, colloc, E);
+  }
+  // dst += A_free src (no zeroing, no constrained rows)
+  void apply64(const double *src, double *dst, hipStream_t s, double *partials = nullptr,
+               const int *skip = nullptr) const {
+    if (patch)
+      launch_patch<double>(patch_op64(), src, dst, s, partials, skip);
+    else
+      launch_apply<double>(p, nq, colloc, B.ptr, D.ptr, G.ptr, src, dst, map.ptr, E, s, partials,
+                           skip);
+  }
+  void apply32(const float *src, float *dst, hipStream_t s) const {
+    if (patch)
+      launch_patch<float>(patch_op32(), src, dst, s);
+    else
+      launch_apply<float>(p, nq, colloc, B32.ptr, D32.ptr, G32.ptr, src, dst, map.ptr, E, s);
   }
 };
 
@@ -93,6 +142,13 @@ static void finish_setup(mfx_op *op, int with_fp32, hipStream_t s) {
     op->G.alloc((uint64_t)op->E * 6 * nq3);
     launch_geometry(op->E, op->cellX.ptr, op->nq, op->xq.ptr, op->wq.ptr, op->G.ptr, s);
   }
+  if (op->patch) {
+    const uint64_t npatch = patch_n_blocks(op->lay);
+    const PatchShape sh = op->shape;
+    const uint64_t P3 = (uint64_t)(sh.bx * op->p + 1) * (sh.by * op->p + 1) * (sh.bz * op->p + 1);
+    op->pmap.alloc(npatch * P3);
+    launch_patch_map(op->lay, op->p, sh, op->pmap.ptr, s);
+  }
   if (with_fp32) {
     if (op->geometry == MFX_GEOM_STORED) {
       op->G32.alloc(op->G.n);
@@ -102,6 +158,10 @@ static void finish_setup(mfx_op *op, int with_fp32, hipStream_t s) {
     launch_to_float(op->B.ptr, op->B32.ptr, op->B.n, s);
     op->D32.alloc(op->D.n);
     launch_to_float(op->D.ptr, op->D32.ptr, op->D.n, s);
+    op->xq32.alloc(op->xq.n);
+    launch_to_float(op->xq.ptr, op->xq32.ptr, op->xq.n, s);
+    op->wq32.alloc(op->wq.n);
+    launch_to_float(op->wq.ptr, op->wq32.ptr, op->wq.n, s);
   }
   MFX_HIP(hipStreamSynchronize(s));
 }
@@ -110,7 +170,8 @@ static void common_init(mfx_op *op, int p, int nq, int quad, int geometry) {
   MFX_REQUIRE(p >= 1 && p <= 8, "degree must be in 1..8");
   MFX_REQUIRE(quad == MFX_QUAD_GAUSS || quad == MFX_QUAD_GLL, "unknown quadrature");
   MFX_REQUIRE(nq == p + 1 || nq == p + 2, "n_q_points_1d must be degree+1 or degree+2");
-  MFX_REQUIRE(geometry == MFX_GEOM_STORED, "only stored geometry is implemented");
+  MFX_REQUIRE(geometry == MFX_GEOM_STORED || geometry == MFX_GEOM_ON_THE_FLY,
+              "unknown geometry mode");
   op->p = p;
   op->nq = nq;
   op->quad = quad;
@@ -125,9 +186,26 @@ static void common_init(mfx_op *op, int p, int nq, int quad, int geometry) {
   op->support.upload(sup.data(), sup.size());
 }
 
+// Metric terms of the lexicographic cell order: the stored array, or a
+// temporary one for on-the-fly operators (set-up kernels only).
+struct GeometryView {
+  const double *G = nullptr;
+  DBuf<double> tmp;
+};
+static void lexicographic_G(const mfx_op *op, GeometryView &v, hipStream_t s) {
+  if (op->G.ptr) {
+    v.G = op->G.ptr;
+    return;
+  }
+  v.tmp.alloc((uint64_t)op->E * 6 * op->nq * op->nq * op->nq);
+  launch_geometry(op->E, op->cellX.ptr, op->nq, op->xq.ptr, op->wq.ptr, v.tmp.ptr, s);
+  v.G = v.tmp.ptr;
+}
+
 extern "C" {
 
 const char *mfx_last_error(void) { return g_last_error.c_str(); }
+int mfx_version(void) { return 100; }
 
 mfx_status mfx_basis_tables(int p, int nq, int quadrature, double *B, double *D, double *xq,
                             double *wq) {
@@ -138,7 +216,6 @@ mfx_status mfx_basis_tables(int p, int nq, int quadrature, double *B, double *D,
     basis_tables(p, nq, quadrature, B, D, xq, wq, sup.data());
   });
 }
-int mfx_version(void) { return 100; }
 
 // ---------------------------------------------------------------- kernels
 mfx_status mfx_bk3_apply_f64(int p, int nq, const double *B, const double *D, const double *G,
@@ -212,6 +289,8 @@ mfx_status mfx_op_create_box(const mfx_box_desc *d, mfx_op **out) {
     *out = nullptr;
     auto op = std::make_unique<mfx_op>();
     common_init(op.get(), d->degree, d->n_q_points_1d, d->quadrature, d->geometry);
+    MFX_REQUIRE(d->geometry == MFX_GEOM_STORED || op->colloc,
+                "on-the-fly geometry is implemented for GLL collocation (nq = p+1)");
     const int p = d->degree;
     uint64_t E = 1, N = 1, Ninner = 1;
     for (int k = 0; k < 3; ++k) {
@@ -222,8 +301,8 @@ mfx_status mfx_op_create_box(const mfx_box_desc *d, mfx_op **out) {
       N *= nk;
       Ninner *= nk - 2;
     }
-    MFX_REQUIRE(E <= 0xFFFFFFFFull / 2, "too many cells");
-    MFX_REQUIRE(N < 0xFFFFFFFFull, "too many DoFs for 32-bit DoF indices");
+    MFX_REQUIRE(E <= 0x7FFFFFFFull, "too many cells");
+    MFX_REQUIRE(N < 0x7FFFFFFFull, "too many DoFs for 31-bit DoF indices");
     op->E = (uint32_t)E;
     op->N = N;
     BoxGeom box;
@@ -251,6 +330,17 @@ mfx_status mfx_op_create_box(const mfx_box_desc *d, mfx_op **out) {
       }
     op->n_cons = cons.size();
     op->cons.upload(cons.data(), cons.size());
+    // stored metric terms stream through the per-cell kernel (measured faster
+    // with G); on-the-fly geometry runs the patch kernel
+    if (op->colloc && d->geometry == MFX_GEOM_ON_THE_FLY) {
+      op->patch = true;
+      op->shape = patch_shape(p);
+      const int b[3] = {op->shape.bx, op->shape.by, op->shape.bz};
+      for (int k = 0; k < 3; ++k) {
+        op->lay.n[k] = d->n_cells[k];
+        op->lay.np[k] = (d->n_cells[k] + b[k] - 1) / b[k];
+      }
+    }
     hipStream_t s = nullptr;
     op->cellX.alloc(E * 24);
     launch_box_cell_vertices(box, op->cellX.ptr, s);
@@ -267,7 +357,9 @@ mfx_status mfx_op_create(const mfx_op_desc *d, mfx_op **out) {
     *out = nullptr;
     MFX_REQUIRE(d->map && d->cell_vertices, "map and cell_vertices are required");
     MFX_REQUIRE(d->n_constrained == 0 || d->constrained, "constrained list missing");
-    MFX_REQUIRE(d->n_dofs < 0xFFFFFFFFull, "too many DoFs for 32-bit DoF indices");
+    MFX_REQUIRE(d->n_dofs < 0x7FFFFFFFull, "too many DoFs for 31-bit DoF indices");
+    MFX_REQUIRE(d->geometry == MFX_GEOM_STORED,
+                "general meshes use stored geometry (on-the-fly needs the box generator)");
     auto op = std::make_unique<mfx_op>();
     common_init(op.get(), d->degree, d->n_q_points_1d, d->quadrature, d->geometry);
     const uint64_t nm3 = (uint64_t)(d->degree + 1) * (d->degree + 1) * (d->degree + 1);
@@ -318,6 +410,11 @@ mfx_status mfx_op_get_info(const mfx_op *op, mfx_op_info *info) {
     info->constrained = op->cons.ptr;
     info->xq = op->xq.ptr;
     info->wq = op->wq.ptr;
+    info->n_patches = op->patch ? patch_n_blocks(op->lay) : 0;
+    info->patch_cells[0] = op->patch ? op->shape.bx : 0;
+    info->patch_cells[1] = op->patch ? op->shape.by : 0;
+    info->patch_cells[2] = op->patch ? op->shape.bz : 0;
+    info->patch_map = op->pmap.ptr;
   });
 }
 
@@ -327,8 +424,7 @@ mfx_status mfx_op_vmult(mfx_op *op, double *dst, const double *src, void *stream
     MFX_REQUIRE(dst != src, "vmult: src and dst must not alias");
     hipStream_t s = as_stream(stream);
     launch_fill(dst, 0.0, op->N, s);
-    launch_apply<double>(op->p, op->nq, op->colloc, op->B.ptr, op->D.ptr, op->G.ptr, src, dst,
-                         op->map.ptr, op->E, s);
+    op->apply64(src, dst, s);
     launch_copy_listed<double>(dst, src, op->cons.ptr, op->n_cons, s);
   });
 }
@@ -340,8 +436,7 @@ mfx_status mfx_op_vmult_f32(mfx_op *op, float *dst, const float *src, void *stre
     MFX_REQUIRE(op->D32.ptr, "operator created without with_fp32");
     hipStream_t s = as_stream(stream);
     launch_fill_f32(dst, 0.0f, op->N, s);
-    launch_apply<float>(op->p, op->nq, op->colloc, op->B32.ptr, op->D32.ptr, op->G32.ptr, src,
-                        dst, op->map.ptr, op->E, s);
+    op->apply32(src, dst, s);
     launch_copy_listed<float>(dst, src, op->cons.ptr, op->n_cons, s);
   });
 }
@@ -350,9 +445,12 @@ mfx_status mfx_op_diagonal(mfx_op *op, double *diag, void *stream) {
   return guard([&] {
     MFX_REQUIRE(op && diag, "null pointer argument");
     hipStream_t s = as_stream(stream);
+    GeometryView gv;
+    lexicographic_G(op, gv, s);
     launch_fill(diag, 0.0, op->N, s);
-    launch_diagonal(op->E, op->p, op->nq, op->B.ptr, op->D.ptr, op->G.ptr, op->map.ptr, diag, s);
+    launch_diagonal(op->E, op->p, op->nq, op->B.ptr, op->D.ptr, gv.G, op->map.ptr, diag, s);
     launch_set_listed(diag, 1.0, op->cons.ptr, op->n_cons, s);
+    if (gv.tmp.ptr) MFX_HIP(hipStreamSynchronize(s));  // temporary geometry freed on return
   });
 }
 
@@ -390,7 +488,7 @@ mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b, const double *in
     MFX_REQUIRE(max_steps >= 0, "max_steps must be >= 0");
     hipStream_t s = as_stream(stream);
     const uint64_t N = op->N;
-    const uint32_t nb_apply = apply_n_blocks(op->p, op->nq, op->colloc, op->E);
+    const uint32_t nb_apply = op->apply_blocks();
     const uint32_t nb_cons = cg_cons_blocks(op->n_cons);
     const uint32_t nb_vec = cg_vec_blocks(N);
     const size_t npart = std::max<size_t>((size_t)nb_apply + nb_cons, 2 * (size_t)nb_vec);
@@ -408,8 +506,7 @@ mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b, const double *in
     const auto t0 = std::chrono::steady_clock::now();
     // r = A x
     launch_fill(op->r.ptr, 0.0, N, s);
-    launch_apply<double>(op->p, op->nq, op->colloc, op->B.ptr, op->D.ptr, op->G.ptr, x,
-                         op->r.ptr, op->map.ptr, op->E, s);
+    op->apply64(x, op->r.ptr, s);
     launch_copy_listed<double>(op->r.ptr, x, op->cons.ptr, op->n_cons, s);
     cg_init(op->st.ptr, b, op->r.ptr, inv_diag, N, op->part.ptr, abs_tol, rel_tol, max_steps, s);
     const int *done = &op->st.ptr->done;
@@ -417,9 +514,7 @@ mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b, const double *in
     for (int it = 0;;) {
       for (int k = 0; k < batch; ++k, ++it) {
         cg_direction(op->st.ptr, op->r.ptr, inv_diag, op->pdir.ptr, op->ap.ptr, N, s);
-        launch_apply<double>(op->p, op->nq, op->colloc, op->B.ptr, op->D.ptr, op->G.ptr,
-                             op->pdir.ptr, op->ap.ptr, op->map.ptr, op->E, s, op->part.ptr,
-                             done);
+        op->apply64(op->pdir.ptr, op->ap.ptr, s, op->part.ptr, done);
         cg_constrained(op->st.ptr, op->cons.ptr, op->n_cons, op->pdir.ptr, op->ap.ptr,
                        op->part.ptr + nb_apply, s);
         cg_alpha(op->st.ptr, op->part.ptr, nb_apply + nb_cons, s);

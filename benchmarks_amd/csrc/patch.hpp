@@ -1,0 +1,39 @@
+// patch.hpp -- data of the patch-of-cells operator kernels (patch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mfx {
+
+struct PatchShape {
+  int bx, by, bz;  // cells per patch and direction
+};
+
+struct PatchLayout {
+  uint32_t n[3];   // cells per direction
+  uint32_t np[3];  // patches per direction
+};
+
+// Device pointers and parameters one patch-kernel launch needs.
+template <typename T>
+struct PatchOp {
+  int p = 0;
+  PatchShape shape{};            // cells per patch the map was built for
+  PatchLayout lay{};
+  const T *D = nullptr;          // collocation derivative [nq][nq]
+  const T *w = nullptr;          // 1D quadrature weights [nq]
+  const T *xq = nullptr;         // 1D quadrature points [nq]
+  const double *cellX = nullptr; // cell vertices [n_cells][8][3], lexicographic cells
+  const uint32_t *pmap = nullptr;
+};
+
+PatchShape patch_shape(int p);
+template <typename T>
+void launch_patch(const PatchOp<T> &o, const T *src, T *dst, hipStream_t s,
+                  double *partials = nullptr, const int *skip = nullptr);
+void launch_patch_map(const PatchLayout &lay, int p, PatchShape sh, uint32_t *pmap, hipStream_t s);
+uint32_t patch_n_blocks(const PatchLayout &lay);
+
+}  // namespace mfx

@@ -52,7 +52,8 @@ class LaplaceOperator:
 
     def __init__(self, degree: int, n_q_points_1d: int, quadrature: int = GAUSS,
                  n_cells=(8, 8, 8), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
-                 deform: float = 0.0, with_fp32: bool = False, _handle=None):
+                 deform: float = 0.0, with_fp32: bool = False, geometry: int = 0,
+                 _handle=None):
         self._h = None
         if _handle is not None:
             self._h = _handle
@@ -61,7 +62,7 @@ class LaplaceOperator:
                 n_cells = (n_cells,) * 3
             d = _lib.BoxDesc()
             d.degree, d.n_q_points_1d, d.quadrature = degree, n_q_points_1d, quadrature
-            d.geometry = _lib.GEOM_STORED
+            d.geometry = geometry
             for k in range(3):
                 d.n_cells[k] = n_cells[k]
                 d.lo[k] = lo[k]

@@ -17,8 +17,8 @@ from collections import defaultdict
 
 
 def short(name: str) -> str:
-    name = re.sub(r"\(.*$", "", name)
     name = name.replace("mfx::(anonymous namespace)::", "").replace("void ", "")
+    name = re.sub(r"\(.*$", "", name)
     return name[:90]
 
 

@@ -175,6 +175,10 @@ typedef struct {
   const double *cell_vertices; /* [n_cells][8][3] */
   const uint32_t *constrained; /* [n_constrained] */
   const double *xq, *wq;       /* 1D quadrature points/weights on [0,1] */
+  /* box meshes with GLL collocation run the patch-of-cells kernels: */
+  uint32_t n_patches;           /* 0 when the per-cell map kernel is used */
+  int patch_cells[3];           /* cells per patch and direction */
+  const uint32_t *patch_map;    /* [n_patches][(bx p+1)(by p+1)(bz p+1)], bit 31 = shared */
 } mfx_op_info;
 mfx_status mfx_op_get_info(const mfx_op *op, mfx_op_info *info);
 

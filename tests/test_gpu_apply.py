@@ -67,6 +67,47 @@ def test_vmult_deformed_matches_oracle(gpu, p, nq, quad):
     assert _rel(yg, P.vmult(x)) <= 1e-13
 
 
+@pytest.mark.parametrize("p", range(1, 9))
+@pytest.mark.parametrize("n", [(3, 4, 5), (1, 1, 1), (5, 2, 3)])
+def test_vmult_on_the_fly_geometry_matches_oracle(gpu, p, n):
+    """Patch kernel with the metric computed from the cell vertices: same
+    operator as the oracle's stored-G one; meshes whose cell counts are not
+    multiples of the patch size leave partial patches."""
+    from benchmarks_amd.laplace import LaplaceOperator
+    print(f"case p={p} n={n} on-the-fly", flush=True)
+    op = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05, geometry=1)
+    P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
+    assert op.info.n_patches > 0 and op.info.G is None
+    x = O.splitmix64_uniform(1, P.N)
+    y = op.initialize_dof_vector()
+    op.vmult(y, torch.from_numpy(x).cuda())
+    yg = y.cpu().numpy()
+    assert _rel(yg, P.vmult(x)) <= TOL64
+    cons = P.constrained.astype(bool)
+    assert np.array_equal(yg[cons], x[cons])
+
+
+@pytest.mark.parametrize("p", [2, 4, 6, 8])
+def test_patch_map_covers_every_dof(gpu, p):
+    """Patch DoF map: every unconstrained DoF appears, constrained ones never;
+    bit 31 marks exactly the DoFs seen by more than one patch."""
+    from benchmarks_amd.laplace import LaplaceOperator
+    op = LaplaceOperator(p, p + 1, O.GLL, n_cells=(5, 4, 3), geometry=1)
+    P = O.make_problem(p, p + 1, O.GLL, (5, 4, 3))
+    bx, by, bz = op.info.patch_cells
+    P3 = (bx * p + 1) * (by * p + 1) * (bz * p + 1)
+    from benchmarks_amd.laplace import _device_view
+    pm = _device_view(op.info.patch_map, torch.int32, op.info.n_patches * P3).cpu().numpy()
+    pm = pm.view(np.uint32)
+    valid = pm[pm != O.INVALID]
+    idx = valid & 0x7FFFFFFF
+    count = np.bincount(idx, minlength=P.N)
+    assert np.array_equal(count > 0, P.constrained == 0)
+    shared = np.zeros(P.N, bool)
+    shared[idx[(valid & 0x80000000) != 0]] = True
+    assert np.array_equal(shared, count > 1)
+
+
 @pytest.mark.parametrize("p,nq,quad", [(2, 3, O.GLL), (4, 5, O.GLL), (4, 6, O.GAUSS),
                                        (8, 9, O.GLL)])
 def test_gather_scatter_bit_exact(gpu, p, nq, quad):
@@ -153,9 +194,12 @@ def test_single_cell_and_all_constrained(gpu):
         assert _rel(yg, P.vmult(x)) <= TOL64
 
 
-@pytest.mark.parametrize("p,nq,quad", [(2, 3, O.GLL), (4, 5, O.GLL), (4, 6, O.GAUSS)])
-def test_vmult_f32(gpu, p, nq, quad):
-    op = _gpu_problem(p, nq, quad, (4, 3, 5), deform=0.05, with_fp32=True)
+@pytest.mark.parametrize("p,nq,quad,geom", [(2, 3, O.GLL, 0), (4, 5, O.GLL, 0), (4, 6, O.GAUSS, 0),
+                                            (4, 5, O.GLL, 1), (8, 9, O.GLL, 1)])
+def test_vmult_f32(gpu, p, nq, quad, geom):
+    from benchmarks_amd.laplace import LaplaceOperator
+    op = LaplaceOperator(p, nq, quad, n_cells=(4, 3, 5), deform=0.05, with_fp32=True,
+                         geometry=geom)
     P = O.make_problem(p, nq, quad, (4, 3, 5), deform=0.05)
     x = O.splitmix64_uniform(11, P.N)
     y = op.initialize_dof_vector(torch.float32)

@@ -71,13 +71,15 @@ def test_bp3_cg_solution_matches_oracle(gpu, cycle):
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-7 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("geom", [0, 1])
 @pytest.mark.parametrize("p", [2, 4, 6, 8])
-def test_bp5_jacobi_cg_matches_oracle(gpu, p):
+def test_bp5_jacobi_cg_matches_oracle(gpu, p, geom):
     """BP5 (GLL collocation) on the deformed cube, random right-hand side with
-    constrained rows zeroed, Jacobi-CG to 1e-8 relative residual."""
+    constrained rows zeroed, Jacobi-CG to 1e-8 relative residual; stored and
+    on-the-fly geometry."""
     from benchmarks_amd.laplace import LaplaceOperator
     n = {2: 9, 4: 5, 6: 3, 8: 3}[p]
-    op = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05)
+    op = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05, geometry=geom)
     P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
     b = O.splitmix64_uniform(1, P.N)
     b[P.constrained.astype(bool)] = 0.0
