@@ -6,9 +6,12 @@ BP5 plain method: scalar Laplacian, Q_p with GLL collocation (q = p+1), unit
 cube deformed by x += 0.05 sin(2 pi x) sin(2 pi y) sin(2 pi z), n^3 cells with
 n = 79 / 53 / 40 (about 32M DoFs each), FP64, input uniform[-1,1] from
 SplitMix64 seed 1.  Value = aggregate GDoF/s = sum_p N_p / sum_p t_p.
-Also reported per degree: the apply kernel's roofline (algorithmic bytes /
-kernel time, from HIP events on the launch stream) and Jacobi-CG
-time-to-1e-8 relative residual.
+
+Each degree is measured with the metric terms stored (G, 48 q^3 B per cell,
+per-cell kernel) and computed on the fly from the cell vertices (patch
+kernel); the faster mode per degree enters the aggregate.  Also reported
+per degree: the apply kernel's time (HIP events on the launch stream), its
+algorithmic bytes and FLOPs, and Jacobi-CG time-to-1e-8 relative residual.
 """
 import argparse
 import ctypes as C
@@ -23,27 +26,43 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12
 FP64_PEAK = 78.6e12
 CELLS = {2: 158, 4: 79, 6: 53, 8: 40}
+STORED, ON_THE_FLY = 0, 1
+# cells per patch of the on-the-fly kernel (patch_shape in benchmarks_amd/csrc/patch.hip)
+PATCH = {1: (4, 4, 4), 2: (3, 3, 3), 3: (2, 2, 4), 4: (4, 2, 1), 5: (2, 2, 1), 6: (2, 1, 1),
+         7: (2, 1, 1), 8: (2, 1, 1)}
 
 
-def algorithmic_bytes(p, N, E):
-    """src read + dst write (8 B each per DoF), uint32 map (4 (p+1)^3 per cell),
-    stored G (6 doubles per quadrature point, q = p+1)."""
+def algorithmic_bytes(p, N, E, geometry, n=None):
+    """Bytes an apply must move at least once: src read + dst write (8 B per
+    DoF each) + the DoF map + the geometry.  Stored: uint32 map (4 (p+1)^3
+    per cell) and G (6 doubles per quadrature point).  On the fly: the patch
+    DoF map (4 B per patch DoF) and 8 vertices per cell (192 B)."""
     q = p + 1
-    return 16 * N + 4 * (p + 1) ** 3 * E + 48 * q ** 3 * E
+    if geometry == STORED:
+        return 16 * N + 4 * (p + 1) ** 3 * E + 48 * q ** 3 * E
+    bx, by, bz = PATCH[p]
+    npatch = -(-n // bx) * -(-n // by) * -(-n // bz)
+    return 16 * N + 4 * (bx * p + 1) * (by * p + 1) * (bz * p + 1) * npatch + 192 * E
 
 
 def flops_per_cell(p):
+    """Sum-factorized BK5 apply: 12 q^4 + 15 q^3 (rooflineBK5.py:23-35 count);
+    the on-the-fly metric arithmetic is not included."""
     q = p + 1
     return 12 * q ** 4 + 15 * q ** 3
 
 
-def bench_degree(p, steps, warmup, do_cg):
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def bench_mode(p, geometry, steps, warmup):
     import torch
 
-    from benchmarks_amd.laplace import (DiagonalMatrix, LaplaceOperator, ReductionControl,
-                                        SolverCG, bk5_apply, random_vector)
+    from benchmarks_amd.laplace import LaplaceOperator, random_vector
     n = CELLS[p]
-    op = LaplaceOperator(p, p + 1, 1, n_cells=n, deform=0.05)
+    op = LaplaceOperator(p, p + 1, 1, n_cells=n, deform=0.05, geometry=geometry)
     N, E = op.m(), op.n_cells()
     x = random_vector(N, 1)
     y = op.initialize_dof_vector()
@@ -51,42 +70,62 @@ def bench_degree(p, steps, warmup, do_cg):
     for _ in range(warmup):
         op.vmult(y, x)
     torch.cuda.synchronize()
-    # whole vmult (what a solver sees)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0, e1 = _events()
     e0.record(stream)
     for _ in range(steps):
         op.vmult(y, x)
     e1.record(stream)
     torch.cuda.synchronize()
     t_vmult = e0.elapsed_time(e1) * 1e-3 / steps
-    # the apply kernel alone (kernel-level entry point, same stream)
-    D = op.device_array("D", torch.float64, (p + 1) ** 2)
-    G = op.device_array("G", torch.float64, E * 6 * (p + 1) ** 3)
-    mp = op.device_array("map", torch.int32, E * (p + 1) ** 3)
+    # the element kernel alone (vmult_add: no zero-fill, no constrained copy)
     scratch = torch.zeros(N, dtype=torch.float64, device="cuda")
     for _ in range(3):
-        bk5_apply(p, D, G, x, scratch, mp, E)
+        op.vmult_add(scratch, x)
     torch.cuda.synchronize()
-    k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    k0, k1 = _events()
     k0.record(stream)
     for _ in range(steps):
-        bk5_apply(p, D, G, x, scratch, mp, E)
+        op.vmult_add(scratch, x)
     k1.record(stream)
     torch.cuda.synchronize()
     t_kernel = k0.elapsed_time(k1) * 1e-3 / steps
-    del D, G, mp, scratch
     # known-size probes for hardware-counter scaling: the deterministic dot
     # reads exactly 16 N bytes, vmult's zero-fill writes exactly 8 N bytes
     from benchmarks_amd._lib import check, lib
     dot = C.c_double()
     check(lib().mfx_vec_dot(C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), N,
                             C.byref(dot), C.c_void_p(stream.cuda_stream)))
-    res = {"p": p, "n_cells_1d": n, "n_dofs": N, "n_cells": E, "ms_vmult": t_vmult * 1e3,
-           "gdofs_vmult": N / t_vmult * 1e-9, "ms_kernel": t_kernel * 1e3,
-           "bytes_alg": algorithmic_bytes(p, N, E),
-           "kernel_GBs": algorithmic_bytes(p, N, E) / t_kernel * 1e-9,
-           "kernel_TFLOPs": flops_per_cell(p) * E / t_kernel * 1e-12}
+    b_alg = algorithmic_bytes(p, N, E, geometry, n)
+    res = {"geometry": "stored" if geometry == STORED else "on_the_fly", "ms_vmult": t_vmult * 1e3,
+           "gdofs_vmult": N / t_vmult * 1e-9, "ms_kernel": t_kernel * 1e3, "bytes_alg": b_alg,
+           "kernel_GBs": b_alg / t_kernel * 1e-9,
+           "kernel_TFLOPs_sumfact": flops_per_cell(p) * E / t_kernel * 1e-12}
+    return op, x, y, res
+
+
+def bench_degree(p, steps, warmup, do_cg, modes):
+    import torch
+
+    from benchmarks_amd.laplace import (DiagonalMatrix, ReductionControl, SolverCG,
+                                        random_vector)
+    out = {"p": p, "n_cells_1d": CELLS[p], "modes": {}}
+    best = None
+    for g in modes:
+        op, x, y, r = bench_mode(p, g, steps, warmup)
+        out["n_dofs"], out["n_cells"] = op.m(), op.n_cells()
+        out["modes"][r["geometry"]] = r
+        if best is None or r["ms_vmult"] < best[1]["ms_vmult"]:
+            best = (g, r)
+        del op, x, y
+        torch.cuda.empty_cache()
+    g, r = best
+    out.update({"geometry": r["geometry"], "ms_vmult": r["ms_vmult"],
+                "gdofs_vmult": r["gdofs_vmult"], "ms_kernel": r["ms_kernel"],
+                "bytes_alg": r["bytes_alg"]})
     if do_cg:
+        from benchmarks_amd.laplace import LaplaceOperator
+        op = LaplaceOperator(p, p + 1, 1, n_cells=CELLS[p], deform=0.05, geometry=g)
+        N = op.m()
         b = random_vector(N, 1)
         op.zero_constrained(b)
         inv = op.compute_inverse_diagonal()
@@ -96,11 +135,11 @@ def bench_degree(p, steps, warmup, do_cg):
         t0 = time.perf_counter()
         SolverCG(ctl).solve(op, sol, b, DiagonalMatrix(inv))
         torch.cuda.synchronize()
-        res.update({"cg_jacobi_its": ctl.last_step(), "cg_jacobi_s": time.perf_counter() - t0,
+        out.update({"cg_jacobi_its": ctl.last_step(), "cg_jacobi_s": time.perf_counter() - t0,
                     "cg_final_rel_res": ctl.last_value() / ctl.initial_value()})
-    del op, x, y
-    torch.cuda.empty_cache()
-    return res
+        del op, b, inv, sol
+        torch.cuda.empty_cache()
+    return out
 
 
 def stream_copy_GBs(n=1 << 28, reps=10):
@@ -109,7 +148,7 @@ def stream_copy_GBs(n=1 << 28, reps=10):
     a = torch.ones(n, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
     b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0, e1 = _events()
     e0.record()
     for _ in range(reps):
         b.copy_(a)
@@ -192,6 +231,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--degrees", default="4,6,8")
+    ap.add_argument("--geometry", choices=["both", "stored", "otf"], default="both")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -200,16 +240,18 @@ def main():
               "exchange) is designed but not built; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     degrees = [int(d) for d in args.degrees.split(",")]
+    modes = {"both": [STORED, ON_THE_FLY], "stored": [STORED], "otf": [ON_THE_FLY]}[args.geometry]
     # host baseline first, before anything initialises the GPU in this process
     cpu = None if args.no_cpu_baseline else cpu_baseline(degrees)
     import torch
     torch.cuda.synchronize()
     stream_copy = stream_copy_GBs()
-    per = [bench_degree(p, args.steps, args.warmup, not args.no_cg) for p in degrees]
+    per = [bench_degree(p, args.steps, args.warmup, not args.no_cg, modes) for p in degrees]
     t_sum = sum(r["ms_vmult"] for r in per) * 1e-3
     n_sum = sum(r["n_dofs"] for r in per)
     dom = max(per, key=lambda r: r["ms_kernel"])
     achieved = dom["bytes_alg"] / (dom["ms_kernel"] * 1e-3)
+    flops = flops_per_cell(dom["p"]) * dom["n_cells"] / (dom["ms_kernel"] * 1e-3)
     out = {
         "metric": "BP5 matvec GDoF/s at p=4-8 (aggregate over p=4,6,8); CG time-to-1e-8 per degree",
         "value": n_sum / t_sum * 1e-9,
@@ -224,17 +266,37 @@ def main():
         "dtype": "f64",
         "data": "synthetic (deformed unit-cube mesh, SplitMix64 seed-1 input)",
         "config": {"workload": "BP5 GLL q=p+1, deformed unit cube, p=4/6/8 on 79^3/53^3/40^3 "
-                               "cells (~32M DoFs each), stored geometry, FP64 vmult",
-                   "degrees": degrees, "parallelism": "single GPU"},
+                               "cells (~32M DoFs each), FP64 vmult; metric terms stored or "
+                               "on the fly, faster per degree",
+                   "degrees": degrees, "geometry": {r["p"]: r["geometry"] for r in per},
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved * 1e-9, "peak": HBM_PEAK * 1e-9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": f"bk5 apply p={dom['p']}", "ms_per_launch": dom["ms_kernel"]},
+                     "kernel": f"BK5 apply p={dom['p']} ({dom['geometry']})",
+                     "ms_per_launch": dom["ms_kernel"],
+                     "fp64_TFLOPs_sumfact": flops * 1e-12, "fp64_frac": flops / FP64_PEAK},
         "per_degree": per,
         "stream_copy_GBs": stream_copy,
     }
+    traffic = measured_traffic(dom)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))
+
+
+def measured_traffic(dom):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/), when one exists for
+    this degree and geometry mode; None otherwise."""
+    path = os.path.join(ROOT, "profiles", f"bk5_apply_bp5_p{dom['p']}_{dom['geometry']}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("APPLY_TRAFFIC_BYTES"):
+            return float(line.split()[-1])
+    return None
 
 
 if __name__ == "__main__":

@@ -429,6 +429,14 @@ mfx_status mfx_op_vmult(mfx_op *op, double *dst, const double *src, void *stream
   });
 }
 
+mfx_status mfx_op_vmult_add(mfx_op *op, double *dst, const double *src, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(op && dst && src, "null pointer argument");
+    MFX_REQUIRE(dst != src, "vmult_add: src and dst must not alias");
+    op->apply64(src, dst, as_stream(stream));
+  });
+}
+
 mfx_status mfx_op_vmult_f32(mfx_op *op, float *dst, const float *src, void *stream) {
   return guard([&] {
     MFX_REQUIRE(op && dst && src, "null pointer argument");

@@ -136,6 +136,12 @@ class LaplaceOperator:
         else:
             raise TypeError("vmult supports float64 and float32 vectors")
 
+    def vmult_add(self, dst: torch.Tensor, src: torch.Tensor) -> None:
+        """dst += A_free src on the unconstrained rows (the element kernel alone)."""
+        self._check_vec(dst)
+        self._check_vec(src)
+        check(lib().mfx_op_vmult_add(self._h, _ptr(dst), _ptr(src), _stream()))
+
     def Tvmult(self, dst, src) -> None:
         self.vmult(dst, src)
 

@@ -58,7 +58,7 @@ def traffic(fetch_csv, write_csv, n_dofs, title="HBM traffic"):
         f, w = fetch.get(k, 0.0), write.get(k, 0.0)
         tot = f * 1024 * f_scale + w * 1024 * w_scale
         out.append(f"| `{short(k)}` | {f:.0f} | {w:.0f} | {tot:.4e} |")
-        if "k_apply" in k:
+        if "k_apply" in k or "k_patch<" in k:
             out.append("")
             out.append(f"APPLY_TRAFFIC_BYTES {short(k)} {tot:.6e}")
             out.append("")

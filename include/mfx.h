@@ -188,6 +188,11 @@ mfx_status mfx_op_get_info(const mfx_op *op, mfx_op_info *info);
  * :397-411).  src and dst must not alias. */
 mfx_status mfx_op_vmult(mfx_op *op, double *dst, const double *src,
                         void *stream);
+/* dst += A_free src over the unconstrained rows only (no zeroing, no
+ * constrained copy): deal.II's vmult_add without the identity rows; the
+ * operator's element kernel on its own. */
+mfx_status mfx_op_vmult_add(mfx_op *op, double *dst, const double *src,
+                            void *stream);
 /* The same in FP32 (geometry rounded to float at set-up; needs with_fp32). */
 mfx_status mfx_op_vmult_f32(mfx_op *op, float *dst, const float *src,
                             void *stream);
