@@ -27,29 +27,32 @@ HBM_PEAK = 8.0e12
 FP64_PEAK = 78.6e12
 CELLS = {2: 158, 4: 79, 6: 53, 8: 40}
 STORED, ON_THE_FLY = 0, 1
-# cells per patch of the on-the-fly kernel (patch_shape in benchmarks_amd/csrc/patch.hip)
-PATCH = {1: (4, 4, 4), 2: (3, 3, 3), 3: (2, 2, 4), 4: (4, 2, 1), 5: (2, 2, 1), 6: (2, 1, 1),
-         7: (2, 1, 1), 8: (2, 1, 1)}
 
 
-def algorithmic_bytes(p, N, E, geometry, n=None):
+def algorithmic_bytes(p, N, E, geometry, patch_cells=None, n_patches=0):
     """Bytes an apply must move at least once: src read + dst write (8 B per
     DoF each) + the DoF map + the geometry.  Stored: uint32 map (4 (p+1)^3
     per cell) and G (6 doubles per quadrature point).  On the fly: the patch
-    DoF map (4 B per patch DoF) and 8 vertices per cell (192 B)."""
+    DoF map (4 B per patch DoF, patches of patch_cells cells) and 8 vertices
+    per cell (192 B)."""
     q = p + 1
     if geometry == STORED:
         return 16 * N + 4 * (p + 1) ** 3 * E + 48 * q ** 3 * E
-    bx, by, bz = PATCH[p]
-    npatch = -(-n // bx) * -(-n // by) * -(-n // bz)
-    return 16 * N + 4 * (bx * p + 1) * (by * p + 1) * (bz * p + 1) * npatch + 192 * E
+    bx, by, bz = patch_cells
+    return 16 * N + 4 * (bx * p + 1) * (by * p + 1) * (bz * p + 1) * n_patches + 192 * E
 
 
-def flops_per_cell(p):
-    """Sum-factorized BK5 apply: 12 q^4 + 15 q^3 (rooflineBK5.py:23-35 count);
-    the on-the-fly metric arithmetic is not included."""
+def flops_per_cell(p, geometry=STORED):
+    """FP64 operations of one cell's apply (an FMA counts 2).  Stored metric:
+    the sum-factorized BK5 count 12 q^4 + 15 q^3 (rooflineBK5.py:23-35:
+    6 q^4 gradient + 6 q^4 transposed gradient + 15 per point for G grad u).
+    On the fly (patch kernel): per point, 73 in place of the 15 (cofactor
+    polynomials 24, det 5, reciprocal 9, weight 2, W 15, three projections
+    18) and per (x,y) column 180 for the trilinear coefficients."""
     q = p + 1
-    return 12 * q ** 4 + 15 * q ** 3
+    if geometry == STORED:
+        return 12 * q ** 4 + 15 * q ** 3
+    return 12 * q ** 4 + 73 * q ** 3 + 180 * q ** 2
 
 
 def _events():
@@ -95,11 +98,13 @@ def bench_mode(p, geometry, steps, warmup):
     dot = C.c_double()
     check(lib().mfx_vec_dot(C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), N,
                             C.byref(dot), C.c_void_p(stream.cuda_stream)))
-    b_alg = algorithmic_bytes(p, N, E, geometry, n)
+    info = op.info
+    b_alg = algorithmic_bytes(p, N, E, geometry, tuple(info.patch_cells), info.n_patches)
+    f_alg = flops_per_cell(p, geometry) * E
     res = {"geometry": "stored" if geometry == STORED else "on_the_fly", "ms_vmult": t_vmult * 1e3,
            "gdofs_vmult": N / t_vmult * 1e-9, "ms_kernel": t_kernel * 1e3, "bytes_alg": b_alg,
-           "kernel_GBs": b_alg / t_kernel * 1e-9,
-           "kernel_TFLOPs_sumfact": flops_per_cell(p) * E / t_kernel * 1e-12}
+           "flops_alg": f_alg, "kernel_GBs": b_alg / t_kernel * 1e-9,
+           "kernel_TFLOPs": f_alg / t_kernel * 1e-12}
     return op, x, y, res
 
 
@@ -121,7 +126,7 @@ def bench_degree(p, steps, warmup, do_cg, modes):
     g, r = best
     out.update({"geometry": r["geometry"], "ms_vmult": r["ms_vmult"],
                 "gdofs_vmult": r["gdofs_vmult"], "ms_kernel": r["ms_kernel"],
-                "bytes_alg": r["bytes_alg"]})
+                "bytes_alg": r["bytes_alg"], "flops_alg": r["flops_alg"]})
     if do_cg:
         from benchmarks_amd.laplace import LaplaceOperator
         op = LaplaceOperator(p, p + 1, 1, n_cells=CELLS[p], deform=0.05, geometry=g)
@@ -250,8 +255,24 @@ def main():
     t_sum = sum(r["ms_vmult"] for r in per) * 1e-3
     n_sum = sum(r["n_dofs"] for r in per)
     dom = max(per, key=lambda r: r["ms_kernel"])
-    achieved = dom["bytes_alg"] / (dom["ms_kernel"] * 1e-3)
-    flops = flops_per_cell(dom["p"]) * dom["n_cells"] / (dom["ms_kernel"] * 1e-3)
+    t_k = dom["ms_kernel"] * 1e-3
+    bw, flops = dom["bytes_alg"] / t_k, dom["flops_alg"] / t_k
+    # the roof that bounds the kernel: FP64 compute when its arithmetic
+    # intensity exceeds the machine balance (78.6 TFLOP/s / 8 TB/s), else HBM
+    intensity = dom["flops_alg"] / dom["bytes_alg"]
+    if intensity > FP64_PEAK / HBM_PEAK:
+        roof = {"bound": "mfma", "achieved": flops * 1e-12, "peak": FP64_PEAK * 1e-12,
+                "unit": "TFLOP/s", "frac": flops / FP64_PEAK}
+    else:
+        roof = {"bound": "hbm", "achieved": bw * 1e-9, "peak": HBM_PEAK * 1e-9,
+                "unit": "GB/s", "frac": bw / HBM_PEAK}
+    roof.update({"traffic": None, "kernel": f"BK5 apply p={dom['p']} ({dom['geometry']})",
+                 "ms_per_launch": dom["ms_kernel"], "flops_per_launch": dom["flops_alg"],
+                 "bytes_per_launch": dom["bytes_alg"], "flop_per_byte": intensity,
+                 "hbm_GBs": bw * 1e-9, "hbm_frac": bw / HBM_PEAK,
+                 "fp64_TFLOPs": flops * 1e-12, "fp64_frac": flops / FP64_PEAK,
+                 "note": "FP64 arithmetic on the vector ALUs; MI355X's FP64 vector and "
+                         "matrix peaks are both 78.6 TFLOP/s"})
     out = {
         "metric": "BP5 matvec GDoF/s at p=4-8 (aggregate over p=4,6,8); CG time-to-1e-8 per degree",
         "value": n_sum / t_sum * 1e-9,
@@ -270,11 +291,7 @@ def main():
                                "on the fly, faster per degree",
                    "degrees": degrees, "geometry": {r["p"]: r["geometry"] for r in per},
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved * 1e-9, "peak": HBM_PEAK * 1e-9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": f"BK5 apply p={dom['p']} ({dom['geometry']})",
-                     "ms_per_launch": dom["ms_kernel"],
-                     "fp64_TFLOPs_sumfact": flops * 1e-12, "fp64_frac": flops / FP64_PEAK},
+        "roofline": roof,
         "per_degree": per,
         "stream_copy_GBs": stream_copy,
     }

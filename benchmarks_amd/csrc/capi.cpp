@@ -81,6 +81,7 @@ struct mfx_op {
   bool patch = false;
   PatchLayout lay{};
   PatchShape shape{};
+  PatchGrid grid64{}, grid32{};
   DBuf<uint32_t> pmap;
   // CG scratch, allocated on first use
   DBuf<double> r, pdir, ap, part;
@@ -95,6 +96,7 @@ struct mfx_op {
     o.p = p;
     o.shape = shape;
     o.lay = lay;
+    o.grid = grid64;
     o.D = D.ptr;
     o.w = wq.ptr;
     o.xq = xq.ptr;
@@ -107,6 +109,7 @@ struct mfx_op {
     o.p = p;
     o.shape = shape;
     o.lay = lay;
+    o.grid = grid32;
     o.D = D32.ptr;
     o.w = wq32.ptr;
     o.xq = xq32.ptr;
@@ -115,7 +118,7 @@ struct mfx_op {
     return o;
   }
   uint32_t apply_blocks() const {
-    return patch ? patch_n_blocks(lay) : apply_n_blocks(p, nq, colloc, E);
+    return patch ? grid64.dot : apply_n_blocks(p, nq, colloc, E);
   }
   // dst += A_free src (no zeroing, no constrained rows)
   void apply64(const double *src, double *dst, hipStream_t s, double *partials = nullptr,
@@ -143,11 +146,13 @@ static void finish_setup(mfx_op *op, int with_fp32, hipStream_t s) {
     launch_geometry(op->E, op->cellX.ptr, op->nq, op->xq.ptr, op->wq.ptr, op->G.ptr, s);
   }
   if (op->patch) {
-    const uint64_t npatch = patch_n_blocks(op->lay);
+    const uint64_t npatch = patch_count(op->lay);
     const PatchShape sh = op->shape;
     const uint64_t P3 = (uint64_t)(sh.bx * op->p + 1) * (sh.by * op->p + 1) * (sh.bz * op->p + 1);
     op->pmap.alloc(npatch * P3);
     launch_patch_map(op->lay, op->p, sh, op->pmap.ptr, s);
+    op->grid64 = patch_grid<double>(op->p, op->lay);
+    if (with_fp32) op->grid32 = patch_grid<float>(op->p, op->lay);
   }
   if (with_fp32) {
     if (op->geometry == MFX_GEOM_STORED) {
@@ -410,7 +415,7 @@ mfx_status mfx_op_get_info(const mfx_op *op, mfx_op_info *info) {
     info->constrained = op->cons.ptr;
     info->xq = op->xq.ptr;
     info->wq = op->wq.ptr;
-    info->n_patches = op->patch ? patch_n_blocks(op->lay) : 0;
+    info->n_patches = op->patch ? patch_count(op->lay) : 0;
     info->patch_cells[0] = op->patch ? op->shape.bx : 0;
     info->patch_cells[1] = op->patch ? op->shape.by : 0;
     info->patch_cells[2] = op->patch ? op->shape.bz : 0;

@@ -16,12 +16,20 @@ struct PatchLayout {
   uint32_t np[3];  // patches per direction
 };
 
+// Workgroups of the persistent patch kernel: as many as the device holds at
+// once (resident workgroups per CU x CUs), at most one per patch.  `dot` is
+// the grid of the variant that also returns per-workgroup p.Ap partials.
+struct PatchGrid {
+  uint32_t plain = 0, dot = 0;
+};
+
 // Device pointers and parameters one patch-kernel launch needs.
 template <typename T>
 struct PatchOp {
   int p = 0;
   PatchShape shape{};            // cells per patch the map was built for
   PatchLayout lay{};
+  PatchGrid grid{};
   const T *D = nullptr;          // collocation derivative [nq][nq]
   const T *w = nullptr;          // 1D quadrature weights [nq]
   const T *xq = nullptr;         // 1D quadrature points [nq]
@@ -30,10 +38,12 @@ struct PatchOp {
 };
 
 PatchShape patch_shape(int p);
+uint32_t patch_count(const PatchLayout &lay);
+template <typename T>
+PatchGrid patch_grid(int p, const PatchLayout &lay);
 template <typename T>
 void launch_patch(const PatchOp<T> &o, const T *src, T *dst, hipStream_t s,
                   double *partials = nullptr, const int *skip = nullptr);
 void launch_patch_map(const PatchLayout &lay, int p, PatchShape sh, uint32_t *pmap, hipStream_t s);
-uint32_t patch_n_blocks(const PatchLayout &lay);
 
 }  // namespace mfx
