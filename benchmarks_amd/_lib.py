@@ -54,6 +54,18 @@ class CgResult(C.Structure):
                 ("last_value", C.c_double), ("seconds", C.c_double)]
 
 
+class MgDesc(C.Structure):
+    _fields_ = [("fine", BoxDesc), ("smoother_degree", C.c_int), ("smoothing_range", C.c_double),
+                ("eig_cg_n_iterations", C.c_int), ("eig_safety", C.c_double),
+                ("coarse_max_dofs", C.c_uint32)]
+
+
+class MgLevelInfo(C.Structure):
+    _fields_ = [("degree", C.c_int), ("n_q_points_1d", C.c_int), ("n_cells", C.c_uint32 * 3),
+                ("n_dofs", C.c_uint64), ("lambda_max", C.c_double), ("lambda_min", C.c_double),
+                ("op", C.c_void_p)]
+
+
 _P = C.c_void_p
 _SIGS = {
     "mfx_last_error": (C.c_char_p, []),
@@ -85,6 +97,17 @@ _SIGS = {
     "mfx_vec_random": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P]),
     "mfx_vec_zero_constrained": (C.c_int, [_P, _P, _P]),
     "mfx_vec_dot": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_double), _P]),
+    "mfx_mg_create": (C.c_int, [C.POINTER(MgDesc), C.POINTER(C.c_void_p)]),
+    "mfx_mg_destroy": (C.c_int, [_P]),
+    "mfx_mg_n_levels": (C.c_int, [_P]),
+    "mfx_mg_get_level": (C.c_int, [_P, C.c_int, C.POINTER(MgLevelInfo)]),
+    "mfx_mg_vmult": (C.c_int, [_P, _P, _P, _P]),
+    "mfx_mg_solve": (C.c_int, [_P, _P, _P, C.c_int, C.c_double, C.c_double,
+                               C.POINTER(CgResult), _P]),
+    "mfx_transfer_prolongate_f64": (C.c_int, [_P, _P, _P, _P, _P]),
+    "mfx_transfer_restrict_f64": (C.c_int, [_P, _P, _P, _P, _P]),
+    "mfx_chebyshev_f32": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double, C.c_int,
+                                    _P]),
 }
 
 _lib = None

@@ -130,6 +130,13 @@ double lagrange_derivative(const double *c, int n, int j, double x) {
 }
 }  // namespace
 
+void lagrange_values(int p, double x, double *values) {
+  MFX_REQUIRE(p >= 1 && p <= 8, "degree must be in 1..8");
+  std::vector<double> sup(p + 1), sw(p + 1);
+  gll_rule(p + 1, sup.data(), sw.data());
+  for (int m = 0; m <= p; ++m) values[m] = lagrange_value(sup.data(), p + 1, m, x);
+}
+
 void basis_tables(int p, int nq, int quad, double *B, double *D, double *xq, double *wq,
                   double *support) {
   MFX_REQUIRE(p >= 1 && p <= 8, "degree must be in 1..8");

@@ -6,138 +6,16 @@
 #include <string>
 #include <vector>
 
-#include "cg_state.hpp"
-#include "mfx_internal.hpp"
-#include "patch.hpp"
+#include "op.hpp"
 
 namespace mfx {
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
-template <typename F>
-static mfx_status guard(F &&f) {
-  try {
-    f();
-    return MFX_OK;
-  } catch (const Error &e) {
-    set_error(e.msg);
-    return e.status;
-  } catch (const std::bad_alloc &) {
-    set_error("host allocation failed");
-    return MFX_ERR_OOM;
-  } catch (const std::exception &e) {
-    set_error(e.what());
-    return MFX_ERR_HIP;
-  }
-}
-
-// Owning device buffer.
-template <typename T>
-struct DBuf {
-  T *ptr = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf &) = delete;
-  DBuf &operator=(const DBuf &) = delete;
-  ~DBuf() { reset(); }
-  void reset() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    n = 0;
-  }
-  void alloc(size_t count) {
-    reset();
-    if (count == 0) return;
-    void *p = nullptr;
-    const hipError_t e = hipMalloc(&p, count * sizeof(T));
-    if (e != hipSuccess)
-      throw Error{e == hipErrorOutOfMemory ? MFX_ERR_OOM : MFX_ERR_HIP,
-                  "hipMalloc(" + std::to_string(count * sizeof(T)) + " B): " +
-                      hipGetErrorString(e)};
-    ptr = static_cast<T *>(p);
-    n = count;
-  }
-  void upload(const T *host, size_t count) {
-    alloc(count);
-    if (count) MFX_HIP(hipMemcpy(ptr, host, count * sizeof(T), hipMemcpyHostToDevice));
-  }
-};
-
 }  // namespace mfx
 
 using namespace mfx;
-
-struct mfx_op {
-  int p = 0, nq = 0, quad = 0, geometry = 0;
-  bool colloc = false;
-  uint32_t E = 0;
-  uint64_t N = 0;
-  uint64_t n_cons = 0;
-  DBuf<uint32_t> map, cons;
-  DBuf<double> G, B, D, xq, wq, support, cellX;
-  DBuf<float> G32, B32, D32, xq32, wq32;
-  // patch-of-cells kernels (box meshes, GLL collocation, on-the-fly geometry)
-  bool patch = false;
-  PatchLayout lay{};
-  PatchShape shape{};
-  PatchGrid grid64{}, grid32{};
-  DBuf<uint32_t> pmap;
-  // CG scratch, allocated on first use
-  DBuf<double> r, pdir, ap, part;
-  DBuf<CgState> st;
-  int *h_done = nullptr;  // pinned
-  ~mfx_op() {
-    if (h_done) (void)hipHostFree(h_done);
-  }
-
-  PatchOp<double> patch_op64() const {
-    PatchOp<double> o;
-    o.p = p;
-    o.shape = shape;
-    o.lay = lay;
-    o.grid = grid64;
-    o.D = D.ptr;
-    o.w = wq.ptr;
-    o.xq = xq.ptr;
-    o.cellX = cellX.ptr;
-    o.pmap = pmap.ptr;
-    return o;
-  }
-  PatchOp<float> patch_op32() const {
-    PatchOp<float> o;
-    o.p = p;
-    o.shape = shape;
-    o.lay = lay;
-    o.grid = grid32;
-    o.D = D32.ptr;
-    o.w = wq32.ptr;
-    o.xq = xq32.ptr;
-    o.cellX = cellX.ptr;
-    o.pmap = pmap.ptr;
-    return o;
-  }
-  uint32_t apply_blocks() const {
-    return patch ? grid64.dot : apply_n_blocks(p, nq, colloc, E);
-  }
-  // dst += A_free src (no zeroing, no constrained rows)
-  void apply64(const double *src, double *dst, hipStream_t s, double *partials = nullptr,
-               const int *skip = nullptr) const {
-    if (patch)
-      launch_patch<double>(patch_op64(), src, dst, s, partials, skip);
-    else
-      launch_apply<double>(p, nq, colloc, B.ptr, D.ptr, G.ptr, src, dst, map.ptr, E, s, partials,
-                           skip);
-  }
-  void apply32(const float *src, float *dst, hipStream_t s) const {
-    if (patch)
-      launch_patch<float>(patch_op32(), src, dst, s);
-    else
-      launch_apply<float>(p, nq, colloc, B32.ptr, D32.ptr, G32.ptr, src, dst, map.ptr, E, s);
-  }
-};
-
-static hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 static void finish_setup(mfx_op *op, int with_fp32, hipStream_t s) {
   const uint64_t nq3 = (uint64_t)op->nq * op->nq * op->nq;
@@ -317,6 +195,8 @@ mfx_status mfx_op_create_box(const mfx_box_desc *d, mfx_op **out) {
       box.hi[k] = d->hi[k];
     }
     box.deform = d->deform;
+    op->is_box = true;
+    op->box = box;
     // constrained DoFs: the boundary nodes of the structured node grid
     const uint64_t Nx = (uint64_t)d->n_cells[0] * p + 1, Ny = (uint64_t)d->n_cells[1] * p + 1,
                    Nz = (uint64_t)d->n_cells[2] * p + 1;

@@ -92,5 +92,8 @@ void gauss_rule(int n, double *x, double *w);
 void gll_rule(int n, double *x, double *w);
 void basis_tables(int p, int nq, int quad, double *B, double *D, double *xq,
                   double *wq, double *support);
+// values at x in [0,1] of the p+1 nodal basis functions on the Gauss-Lobatto
+// support points (the FE_Q basis of degree p)
+void lagrange_values(int p, double x, double *values);
 
 }  // namespace mfx

@@ -53,8 +53,10 @@ class LaplaceOperator:
     def __init__(self, degree: int, n_q_points_1d: int, quadrature: int = GAUSS,
                  n_cells=(8, 8, 8), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
                  deform: float = 0.0, with_fp32: bool = False, geometry: int = 0,
-                 _handle=None):
+                 _handle=None, _owner=None):
         self._h = None
+        # a handle owned by another object (a multigrid level) is not destroyed here
+        self._owner = _owner
         if _handle is not None:
             self._h = _handle
         else:
@@ -105,6 +107,8 @@ class LaplaceOperator:
         return cls(degree, n_q_points_1d, quadrature, _handle=h)
 
     def __del__(self):
+        if getattr(self, "_owner", None) is not None:
+            return
         if getattr(self, "_h", None) is not None and lib is not None:
             try:
                 lib().mfx_op_destroy(self._h)
@@ -229,9 +233,90 @@ class DiagonalMatrix:
         return self.vector
 
 
+class PreconditionMG:
+    """Mixed-precision p/h-multigrid preconditioner (mfx_mg_*): FP32 V-cycle
+    with Chebyshev(Jacobi) smoothing, nodal-interpolation transfers and an
+    FP64 direct solve on the coarsest level.  No counterpart in the
+    reference, whose multigrid include is unused (CEED_bp/src/bp3.cc:27);
+    the parameters are those of deal.II's PreconditionChebyshev.  The fine
+    operator (level 0) is built here: operator() returns it."""
+
+    def __init__(self, degree: int, n_q_points_1d: int, quadrature: int = GAUSS,
+                 n_cells=(8, 8, 8), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform: float = 0.0,
+                 geometry: int = 0, smoother_degree: int = 5, smoothing_range: float = 15.0,
+                 eig_cg_n_iterations: int = 12, eig_safety: float = 1.2,
+                 coarse_max_dofs: int = 2000):
+        if isinstance(n_cells, int):
+            n_cells = (n_cells,) * 3
+        d = _lib.MgDesc()
+        f = d.fine
+        f.degree, f.n_q_points_1d, f.quadrature, f.geometry = degree, n_q_points_1d, quadrature, geometry
+        for k in range(3):
+            f.n_cells[k], f.lo[k], f.hi[k] = n_cells[k], lo[k], hi[k]
+        f.deform = deform
+        f.with_fp32 = 1
+        d.smoother_degree, d.smoothing_range = smoother_degree, smoothing_range
+        d.eig_cg_n_iterations, d.eig_safety = eig_cg_n_iterations, eig_safety
+        d.coarse_max_dofs = coarse_max_dofs
+        self._h = None
+        h = C.c_void_p()
+        check(lib().mfx_mg_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self._levels = [self._level(i) for i in range(self.n_levels())]
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and lib is not None:
+            try:
+                lib().mfx_mg_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def n_levels(self) -> int:
+        return int(lib().mfx_mg_n_levels(self._h))
+
+    def _level(self, i):
+        info = _lib.MgLevelInfo()
+        check(lib().mfx_mg_get_level(self._h, i, C.byref(info)))
+        op = LaplaceOperator(0, 0, _handle=C.c_void_p(info.op), _owner=self)
+        return {"degree": info.degree, "n_q_points_1d": info.n_q_points_1d,
+                "n_cells": tuple(info.n_cells), "n_dofs": int(info.n_dofs),
+                "lambda_max": info.lambda_max, "lambda_min": info.lambda_min, "operator": op}
+
+    def level(self, i: int) -> dict:
+        return self._levels[i]
+
+    def operator(self) -> LaplaceOperator:
+        return self._levels[0]["operator"]
+
+    def vmult(self, dst: torch.Tensor, src: torch.Tensor) -> None:
+        """dst = one V-cycle applied to src (FP64 in and out)."""
+        check(lib().mfx_mg_vmult(self._h, _ptr(dst), _ptr(src), _stream()))
+
+
+def transfer_prolongate(fine: LaplaceOperator, coarse: LaplaceOperator, src: torch.Tensor):
+    dst = fine.initialize_dof_vector()
+    check(lib().mfx_transfer_prolongate_f64(fine._h, coarse._h, _ptr(dst), _ptr(src), _stream()))
+    return dst
+
+
+def transfer_restrict(fine: LaplaceOperator, coarse: LaplaceOperator, src: torch.Tensor):
+    dst = coarse.initialize_dof_vector()
+    check(lib().mfx_transfer_restrict_f64(fine._h, coarse._h, _ptr(dst), _ptr(src), _stream()))
+    return dst
+
+
+def chebyshev_f32(op: LaplaceOperator, inv_diag: torch.Tensor, x: torch.Tensor, b: torch.Tensor,
+                  degree: int, lambda_min: float, lambda_max: float, x_is_zero: bool) -> None:
+    """x receives `degree` Chebyshev(Jacobi) steps towards A x = b (FP32)."""
+    check(lib().mfx_chebyshev_f32(op._h, _ptr(inv_diag), _ptr(x), _ptr(b), degree, lambda_min,
+                                  lambda_max, int(x_is_zero), _stream()))
+
+
 class SolverCG:
-    """deal.II SolverCG<Vector>(control).solve(A, x, b, preconditioner); the
-    whole iteration runs on the GPU (mfx_cg_solve)."""
+    """deal.II SolverCG<Vector>(control).solve(A, x, b, preconditioner).  With
+    identity or Jacobi preconditioning the whole iteration runs on the GPU
+    (mfx_cg_solve); with PreconditionMG it is the FP64 CG of mfx_mg_solve."""
 
     def __init__(self, control: ReductionControl):
         self.control = control
@@ -239,16 +324,28 @@ class SolverCG:
     def solve(self, A: LaplaceOperator, x: torch.Tensor, b: torch.Tensor, preconditioner) -> None:
         A._check_vec(x)
         A._check_vec(b)
+        res = _lib.CgResult()
+        if isinstance(preconditioner, PreconditionMG):
+            if A._h.value != preconditioner.operator()._h.value:
+                raise ValueError("A must be the multigrid's fine operator (PreconditionMG.operator())")
+            st = lib().mfx_mg_solve(preconditioner._h, _ptr(x), _ptr(b), self.control.max_steps,
+                                    self.control.tolerance, self.control.reduction, C.byref(res),
+                                    _stream())
+            self._record(res, st)
+            return
         if isinstance(preconditioner, DiagonalMatrix):
             inv = _ptr(preconditioner.vector)
         elif isinstance(preconditioner, PreconditionIdentity) or preconditioner is None:
             inv = None
         else:
-            raise TypeError("preconditioner must be PreconditionIdentity or DiagonalMatrix")
-        res = _lib.CgResult()
+            raise TypeError("preconditioner must be PreconditionIdentity, DiagonalMatrix or "
+                            "PreconditionMG")
         st = lib().mfx_cg_solve(A._h, _ptr(x), _ptr(b), inv, self.control.max_steps,
                                 self.control.tolerance, self.control.reduction, C.byref(res),
                                 _stream())
+        self._record(res, st)
+
+    def _record(self, res, st):
         c = self.control
         c._last_step, c._initial, c._last, c.seconds = (res.steps, res.initial_value,
                                                         res.last_value, res.seconds)

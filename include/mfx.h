@@ -243,6 +243,70 @@ mfx_status mfx_vec_zero_constrained(mfx_op *op, double *x, void *stream);
 mfx_status mfx_vec_dot(const double *x, const double *y, uint64_t n,
                        double *host_result, void *stream);
 
+/* ----------------------------------------------------------------------
+ * Mixed-precision multigrid.  No counterpart in the reference, which solves
+ * with identity/Jacobi CG only (CEED_bp/src/bp3.cc:264-288; its multigrid
+ * include at CEED_bp/src/bp3.cc:27 is unused).  The parameters follow
+ * deal.II's PreconditionChebyshev::AdditionalData and the p/h-multigrid
+ * preconditioners of deal.II's matrix-free solvers.
+ *
+ * Levels: Q_p, Q_{p/2}, ..., Q_1 on the fine cells, then Q_1 on n/2, n/4, ...
+ * cells per direction while the free DoFs exceed coarse_max_dofs (every
+ * halving needs even cell counts).  The coarsest level is solved directly in
+ * FP64; every other level smooths with Chebyshev(Jacobi) in FP32.  Transfers
+ * are nodal interpolation P and its transpose R = P^T.
+ * ---------------------------------------------------------------------- */
+typedef struct mfx_mg mfx_mg;
+
+typedef struct {
+  mfx_box_desc fine;        /* mesh, degree, quadrature, geometry of the fine level */
+  int smoother_degree;      /* Chebyshev steps per smoothing; 0: 5 */
+  double smoothing_range;   /* lambda_min = lambda_max / range; 0: 15 */
+  int eig_cg_n_iterations;  /* CG (Lanczos) steps of the lambda_max estimate; 0: 12 */
+  double eig_safety;        /* lambda_max = safety x estimate; 0: 1.2 */
+  uint32_t coarse_max_dofs; /* largest free-DoF count solved directly; 0: 2000 */
+} mfx_mg_desc;
+
+typedef struct {
+  int degree, n_q_points_1d;
+  uint32_t n_cells[3];
+  uint64_t n_dofs;
+  double lambda_max, lambda_min; /* Chebyshev interval (0 on the coarsest level) */
+  mfx_op *op;                    /* level operator, owned by the multigrid object */
+} mfx_mg_level_info;
+
+mfx_status mfx_mg_create(const mfx_mg_desc *desc, mfx_mg **out);
+mfx_status mfx_mg_destroy(mfx_mg *mg);
+int mfx_mg_n_levels(const mfx_mg *mg);
+/* level 0 is the fine level; its operator is the FP64 operator of the solve */
+mfx_status mfx_mg_get_level(const mfx_mg *mg, int level, mfx_mg_level_info *info);
+/* Preconditioner application (deal.II PreconditionMG::vmult): dst = one
+ * FP32 V-cycle from a zero guess applied to src; constrained rows pass
+ * through.  FP64 in and out. */
+mfx_status mfx_mg_vmult(mfx_mg *mg, double *dst, const double *src, void *stream);
+/* FP64 CG preconditioned by one V-cycle per step, on the fine operator; same
+ * stopping rule and result fields as mfx_cg_solve.  Synchronous. */
+mfx_status mfx_mg_solve(mfx_mg *mg, double *x, const double *b, int max_steps,
+                        double abs_tol, double rel_tol, mfx_cg_result *result,
+                        void *stream);
+
+/* Transfers between two operators made by mfx_op_create_box on the same box:
+ * same cells and lower degree (p-transfer), or same degree and half the
+ * cells per direction (h-transfer).  Vectors are FP64 on the device. */
+mfx_status mfx_transfer_prolongate_f64(const mfx_op *fine, const mfx_op *coarse,
+                                       double *dst_fine, const double *src_coarse,
+                                       void *stream);
+mfx_status mfx_transfer_restrict_f64(const mfx_op *fine, const mfx_op *coarse,
+                                     double *dst_coarse, const double *src_fine,
+                                     void *stream);
+/* Chebyshev smoother steps on an FP32 operator (created with with_fp32): x
+ * receives `degree` Chebyshev(Jacobi) updates towards A x = b on the
+ * interval [lambda_min, lambda_max]; x_is_zero skips the first operator
+ * application.  Synchronous. */
+mfx_status mfx_chebyshev_f32(mfx_op *op, const float *inv_diag, float *x, const float *b,
+                             int degree, double lambda_min, double lambda_max,
+                             int x_is_zero, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
