@@ -4,14 +4,20 @@ Workload (one "step"): one operator apply (vmult, deal.II semantics: dst = 0,
 gather/apply/scatter, constrained rows copied) for each of p = 4, 6, 8 of the
 BP5 plain method: scalar Laplacian, Q_p with GLL collocation (q = p+1), unit
 cube deformed by x += 0.05 sin(2 pi x) sin(2 pi y) sin(2 pi z), n^3 cells with
-n = 79 / 53 / 40 (about 32M DoFs each), FP64, input uniform[-1,1] from
-SplitMix64 seed 1.  Value = aggregate GDoF/s = sum_p N_p / sum_p t_p.
+n = 80 / 52 / 40 (33.1M / 30.7M / 33.1M DoFs), FP64, input uniform[-1,1] from
+SplitMix64 seed 1.  Value = aggregate GDoF/s = sum_p N_p / sum_p t_p.  (The
+cell counts have enough factors of two for the multigrid hierarchy to reach a
+directly solvable coarse mesh.)
 
 Each degree is measured with the metric terms stored (G, 48 q^3 B per cell,
 per-cell kernel) and computed on the fly from the cell vertices (patch
 kernel); the faster mode per degree enters the aggregate.  Also reported
 per degree: the apply kernel's time (HIP events on the launch stream), its
-algorithmic bytes and FLOPs, and Jacobi-CG time-to-1e-8 relative residual.
+algorithmic bytes and FLOPs, Jacobi-CG and mixed-precision multigrid CG
+time-to-1e-8 relative residual on the same operator and right-hand side, and
+the mixed-precision BP3 solve (Q4, Gauss q=6, deformed 128^3 cells, 135M
+DoFs, manufactured-solution right-hand side, FP64 CG to 1e-10 with an FP32
+p/h-multigrid V-cycle) with its error against an FP64 solution.
 """
 import argparse
 import ctypes as C
@@ -25,7 +31,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
 FP64_PEAK = 78.6e12
-CELLS = {2: 158, 4: 79, 6: 53, 8: 40}
+CELLS = {2: 160, 4: 80, 6: 52, 8: 40}
 STORED, ON_THE_FLY = 0, 1
 
 
@@ -142,8 +148,103 @@ def bench_degree(p, steps, warmup, do_cg, modes):
         torch.cuda.synchronize()
         out.update({"cg_jacobi_its": ctl.last_step(), "cg_jacobi_s": time.perf_counter() - t0,
                     "cg_final_rel_res": ctl.last_value() / ctl.initial_value()})
+        out.update(mg_solve(p, p + 1, 1, CELLS[p], ON_THE_FLY, b, sol, 1e-8))
         del op, b, inv, sol
         torch.cuda.empty_cache()
+    return out
+
+
+def _timed_solve(A, x, b, precond, rel_tol, max_steps):
+    import torch
+
+    from benchmarks_amd.laplace import ReductionControl, SolverCG
+    ctl = ReductionControl(max_steps, 1e-300, rel_tol)
+    x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    SolverCG(ctl).solve(A, x, b, precond)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, ctl
+
+
+def _rel(a, b):
+    import torch
+    return (torch.linalg.norm(a - b) / torch.linalg.norm(b)).item()
+
+
+def mg_solve(p, nq, quad, n, geometry, b, x_ref_jacobi, rel_tol):
+    """FP64 CG preconditioned by the FP32 p/h-multigrid V-cycle on the same
+    problem as a Jacobi-CG solve: set-up time, time and steps to rel_tol, and
+    the errors of both solutions against an FP64 solution whose residual is
+    certified to 1e-13 by the FP64 outer CG."""
+    import torch
+
+    from benchmarks_amd.laplace import PreconditionMG
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mg = PreconditionMG(p, nq, quad, n, deform=0.05, geometry=geometry)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    A = mg.operator()
+    x = A.initialize_dof_vector()
+    _timed_solve(A, x, b, mg, rel_tol, 500)  # warm-up (kernel loading)
+    t_mg, ctl = _timed_solve(A, x, b, mg, rel_tol, 500)
+    x_ref = A.initialize_dof_vector()
+    _timed_solve(A, x_ref, b, mg, 1e-13, 200)
+    res = {"mg_levels": [(mg.level(l)["degree"], mg.level(l)["n_cells"][0])
+                         for l in range(mg.n_levels())],
+           "mg_setup_s": t_setup, "cg_mg_its": ctl.last_step(), "cg_mg_s": t_mg,
+           "cg_mg_final_rel_res": ctl.last_value() / ctl.initial_value(),
+           "cg_mg_err_vs_fp64": _rel(x, x_ref),
+           "cg_jacobi_err_vs_fp64": _rel(x_ref_jacobi, x_ref)}
+    del mg, A, x, x_ref
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_c3(n=128, p=4, nq=6):
+    """Mixed-precision BP3 solve: Q4, Gauss q=6, deformed n^3 cells,
+    f = 3 pi^2 prod sin(pi x_d), u = 0 on the boundary; FP64 CG with the FP32
+    multigrid V-cycle to 1e-10 (and 1e-8), Jacobi-CG to 1e-8 on the same
+    operator; errors against an FP64 solution certified to 1e-13 and, at the
+    nodes, against the exact solution prod sin(pi x_d)."""
+    import math
+
+    import torch
+
+    from benchmarks_amd.laplace import DiagonalMatrix, PreconditionMG
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mg = PreconditionMG(p, nq, 0, n, deform=0.05, geometry=STORED)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    A = mg.operator()
+    b = A.rhs(1)
+    A.zero_constrained(b)
+    x = A.initialize_dof_vector()
+    _timed_solve(A, x, b, mg, 1e-6, 500)  # warm-up
+    t8, c8 = _timed_solve(A, x, b, mg, 1e-8, 500)
+    t10, c10 = _timed_solve(A, x, b, mg, 1e-10, 500)
+    x_ref = A.initialize_dof_vector()
+    _timed_solve(A, x_ref, b, mg, 1e-13, 200)
+    xyz = A.dof_coordinates()
+    u = torch.sin(math.pi * xyz[:, 0]) * torch.sin(math.pi * xyz[:, 1]) * torch.sin(math.pi * xyz[:, 2])
+    err_exact = ((x - u).abs().max() / u.abs().max()).item()
+    del xyz, u
+    inv = A.compute_inverse_diagonal()
+    xj = A.initialize_dof_vector()
+    tj, cj = _timed_solve(A, xj, b, DiagonalMatrix(inv), 1e-8, 100000)
+    out = {"workload": f"BP3 Q{p} Gauss q={nq}, deformed {n}^3 cells, manufactured sin RHS",
+           "n_dofs": A.m(), "mg_levels": [(mg.level(l)["degree"], mg.level(l)["n_cells"][0])
+                                          for l in range(mg.n_levels())],
+           "mg_setup_s": t_setup, "cg_mg_its_1e-10": c10.last_step(), "cg_mg_s_1e-10": t10,
+           "cg_mg_its_1e-8": c8.last_step(), "cg_mg_s_1e-8": t8,
+           "cg_jacobi_its_1e-8": cj.last_step(), "cg_jacobi_s_1e-8": tj,
+           "speedup_1e-8": tj / t8,
+           "cg_mg_err_vs_fp64_1e-10": _rel(x, x_ref), "cg_jacobi_err_vs_fp64_1e-8": _rel(xj, x_ref),
+           "nodal_max_err_vs_exact": err_exact}
+    del mg, A, b, x, x_ref, inv, xj
+    torch.cuda.empty_cache()
     return out
 
 
@@ -239,6 +340,7 @@ def main():
     ap.add_argument("--geometry", choices=["both", "stored", "otf"], default="both")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 135M-DoF mixed-precision solve")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the multi-GPU path (slab decomposition with halo "
@@ -286,8 +388,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (deformed unit-cube mesh, SplitMix64 seed-1 input)",
-        "config": {"workload": "BP5 GLL q=p+1, deformed unit cube, p=4/6/8 on 79^3/53^3/40^3 "
-                               "cells (~32M DoFs each), FP64 vmult; metric terms stored or "
+        "config": {"workload": "BP5 GLL q=p+1, deformed unit cube, p=4/6/8 on 80^3/52^3/40^3 "
+                               "cells (33.1M / 30.7M / 33.1M DoFs), FP64 vmult; metric terms stored or "
                                "on the fly, faster per degree",
                    "degrees": degrees, "geometry": {r["p"]: r["geometry"] for r in per},
                    "parallelism": "single GPU"},
@@ -298,6 +400,8 @@ def main():
     traffic = measured_traffic(dom)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
+    if not args.no_cg and not args.no_c3:
+        out["mixed_precision_bp3"] = bench_c3()
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))

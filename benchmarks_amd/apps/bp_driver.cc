@@ -11,6 +11,12 @@
 //   bp_driver <degree> [minsize] [maxsize] [nq]      (nq defaults to degree+2)
 //   bp_driver --bp5 <degree> <n> [reps]              BP5: GLL, deformed unit cube,
 //                                                    Jacobi-CG to 1e-8 from a random rhs
+//   bp_driver --mg <degree> <n> [nq]                 mixed-precision BP3: Gauss nq
+//                                                    (degree+2), deformed unit cube,
+//                                                    f = 3 pi^2 prod sin(pi x_d), FP64 CG
+//                                                    to 1e-10 preconditioned by the FP32
+//                                                    multigrid V-cycle, next to
+//                                                    Jacobi-CG to 1e-8
 //   bp_driver --verify <bk3|bk5> <nq> <nelmt>        the verification fixture of
 //       CEED_BK/src/BK3/serial_verification.cc:25-79 and
 //       sum_factorization/tests/BK5/serial_verification.cc:16-47 (G = 2, in = 3,
@@ -133,6 +139,59 @@ static int run_bp5(unsigned degree, unsigned n, int reps) {
   return 0;
 }
 
+// Mixed-precision multigrid CG against Jacobi-CG on one BP3 problem.
+static int run_mg(unsigned degree, unsigned n, unsigned nq) {
+  mfx::BoxMesh mesh;
+  for (int d = 0; d < 3; ++d) mesh.n_cells[d] = n;
+  mesh.deform = 0.05;
+  check_hip_sync();
+  auto t0 = Clock::now();
+  mfx::PreconditionMG mg(mesh, degree, nq, mfx::Quadrature::gauss);
+  check_hip_sync();
+  const double t_setup = seconds_since(t0);
+  const auto &op = mg.get_operator();
+  VectorD b, x, xj;
+  op.compute_rhs(b, 1);
+  mfx::check(mfx_vec_zero_constrained(op.handle(), b.get_values(), nullptr));
+  op.initialize_dof_vector(x);
+  op.initialize_dof_vector(xj);
+  std::printf("levels:");
+  for (unsigned l = 0; l < mg.n_levels(); ++l) {
+    const auto info = mg.level(l);
+    std::printf(" Q%d/%u^3", info.degree, info.n_cells[0]);
+  }
+  std::printf("\n");
+  double t_mg = 0.0;
+  unsigned its_mg = 0;
+  for (int r = 0; r < 2; ++r) {  // the first solve loads the kernels
+    mfx::ReductionControl control(200, 1e-300, 1e-10);
+    mfx::SolverCG<VectorD> cg(control);
+    x = 0.;
+    check_hip_sync();
+    t0 = Clock::now();
+    cg.solve(op, x, b, mg);
+    t_mg = seconds_since(t0);
+    its_mg = control.last_step();
+  }
+  mfx::DiagonalMatrix<VectorD> jacobi;
+  op.compute_inverse_diagonal(jacobi.get_vector());
+  mfx::ReductionControl cj(100000, 1e-300, 1e-8);
+  mfx::SolverCG<VectorD> cg(cj);
+  check_hip_sync();
+  t0 = Clock::now();
+  cg.solve(op, xj, b, jacobi);
+  const double t_j = seconds_since(t0);
+  VectorD diff = x;
+  std::vector<double> hx = x.to_host(), hj = xj.to_host();
+  for (std::size_t i = 0; i < hx.size(); ++i) hx[i] -= hj[i];
+  diff.import_elements(hx);
+  std::printf("RESULT mg p=%u cells=%u dofs=%llu setup_s=%.6e mg_its=%u mg_s=%.6e jacobi_its=%u "
+              "jacobi_s=%.6e rel_diff=%.6e\n",
+              degree, op.n_cells(), (unsigned long long)op.m(), t_setup, its_mg, t_mg,
+              cj.last_step(), t_j, diff.l2_norm() / xj.l2_norm());
+  return 0;
+}
+
 // E-vector kernels on the reference's verification fixture.
 static int run_verify(const std::string &kind, unsigned nq, unsigned nelmt) {
   const bool bk5 = kind == "bk5";
@@ -167,6 +226,14 @@ int main(int argc, char **argv) {
         return 1;
       }
       return run_verify(argv[2], std::atoi(argv[3]), std::atoi(argv[4]));
+    }
+    if (argc > 1 && std::string(argv[1]) == "--mg") {
+      if (argc < 4) {
+        std::printf("Usage: bp_driver --mg degree n [nq]\n");
+        return 1;
+      }
+      const unsigned degree = std::atoi(argv[2]);
+      return run_mg(degree, std::atoi(argv[3]), argc > 4 ? std::atoi(argv[4]) : degree + 2);
     }
     if (argc > 1 && std::string(argv[1]) == "--bp5") {
       if (argc < 4) {

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mg.hpp"
@@ -301,21 +302,33 @@ void spd_inverse(std::vector<double> &A, int n) {
       L[(size_t)i * n + j] = v / ljj;
     }
   }
-  // columns of A^-1: solve L L^T x = e_c
-  std::vector<double> y(n);
-  for (int c = 0; c < n; ++c) {
-    for (int i = 0; i < n; ++i) {
-      double v = (i == c) ? 1.0 : 0.0;
-      for (int k = 0; k < i; ++k) v -= L[(size_t)i * n + k] * y[k];
-      y[i] = v / L[(size_t)i * n + i];
+  // columns of A^-1: solve L L^T x = e_c (y_i = 0 for i < c in the forward
+  // solve); the columns are independent and split over host threads
+  auto columns = [&](int c0, int c1) {
+    std::vector<double> y(n);
+    for (int c = c0; c < c1; ++c) {
+      std::fill(y.begin(), y.begin() + c, 0.0);
+      for (int i = c; i < n; ++i) {
+        double v = (i == c) ? 1.0 : 0.0;
+        for (int k = c; k < i; ++k) v -= L[(size_t)i * n + k] * y[k];
+        y[i] = v / L[(size_t)i * n + i];
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        double v = y[i];
+        for (int k = i + 1; k < n; ++k) v -= L[(size_t)k * n + i] * y[k];
+        y[i] = v / L[(size_t)i * n + i];
+      }
+      for (int i = 0; i < n; ++i) A[(size_t)i * n + c] = y[i];
     }
-    for (int i = n - 1; i >= 0; --i) {
-      double v = y[i];
-      for (int k = i + 1; k < n; ++k) v -= L[(size_t)k * n + i] * y[k];
-      y[i] = v / L[(size_t)i * n + i];
-    }
-    for (int i = 0; i < n; ++i) A[(size_t)i * n + c] = y[i];
+  };
+  const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 256 || nt == 1) {
+    columns(0, n);
+    return;
   }
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t) pool.emplace_back(columns, (int)((int64_t)n * t / nt), (int)((int64_t)n * (t + 1) / nt));
+  for (auto &th : pool) th.join();
 }
 
 }  // namespace

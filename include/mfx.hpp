@@ -9,7 +9,9 @@
 //       (device vector: size, get_values, operator=(0), l2_norm, import);
 //   SolverCG, ReductionControl, PreconditionIdentity, DiagonalMatrix,
 //   SolverControl::NoConvergence (CEED_bp/src/bp3.cc:264-288,
-//   bp5_kokkos/benchmark.cc:336-378).
+//   bp5_kokkos/benchmark.cc:336-378);
+//   PreconditionMG: the mixed-precision p/h-multigrid of mfx_mg_* (no
+//   counterpart in the reference; deal.II's PreconditionChebyshev parameters).
 // Errors become exceptions (mfx::Error / NoConvergence), as deal.II's do.
 #pragma once
 
@@ -150,7 +152,12 @@ class LaplaceOperator {
   LaplaceOperator(const LaplaceOperator &) = delete;
   LaplaceOperator &operator=(const LaplaceOperator &) = delete;
   ~LaplaceOperator() {
-    if (op_) mfx_op_destroy(op_);
+    if (op_ && owned_) mfx_op_destroy(op_);
+  }
+  // view of an operator owned elsewhere (a multigrid level)
+  static LaplaceOperator view(mfx_op *op) { return LaplaceOperator(op); }
+  LaplaceOperator(LaplaceOperator &&o) noexcept : op_(o.op_), owned_(o.owned_) {
+    o.op_ = nullptr;
   }
 
   // dst = A src; constrained rows are the identity (copy_constrained_values).
@@ -180,7 +187,72 @@ class LaplaceOperator {
   mfx_op *handle() const { return op_; }
 
  private:
+  explicit LaplaceOperator(mfx_op *op) : op_(op), owned_(false) {}
   mfx_op *op_ = nullptr;
+  bool owned_ = true;
+};
+
+// Multigrid preconditioner: an FP32 V-cycle (Chebyshev smoothing, nodal
+// transfers, FP64 coarse solve) over the levels Q_p ... Q_1 and h-coarsened
+// Q_1 meshes.  It builds the fine operator itself; solve with
+// SolverCG::solve(mg.get_operator(), x, b, mg).
+struct MGAdditionalData {
+  unsigned smoother_degree = 5;         // Chebyshev steps per smoothing
+  double smoothing_range = 15.0;        // lambda_min = lambda_max / range
+  unsigned eig_cg_n_iterations = 12;    // CG (Lanczos) steps of the estimate
+  double eig_safety = 1.2;              // lambda_max = safety x estimate
+  std::uint32_t coarse_max_dofs = 2000; // largest directly solved level
+};
+
+class PreconditionMG {
+ public:
+  using AdditionalData = MGAdditionalData;
+  PreconditionMG(const BoxMesh &mesh, unsigned fe_degree, unsigned n_q_points_1d,
+                 Quadrature quad = Quadrature::gauss, const AdditionalData &data = {},
+                 int geometry = MFX_GEOM_STORED) {
+    mfx_mg_desc d{};
+    d.fine.degree = static_cast<int>(fe_degree);
+    d.fine.n_q_points_1d = static_cast<int>(n_q_points_1d);
+    d.fine.quadrature = static_cast<int>(quad);
+    d.fine.geometry = geometry;
+    for (int k = 0; k < 3; ++k) {
+      d.fine.n_cells[k] = mesh.n_cells[k];
+      d.fine.lo[k] = mesh.p1[k];
+      d.fine.hi[k] = mesh.p2[k];
+    }
+    d.fine.deform = mesh.deform;
+    d.smoother_degree = static_cast<int>(data.smoother_degree);
+    d.smoothing_range = data.smoothing_range;
+    d.eig_cg_n_iterations = static_cast<int>(data.eig_cg_n_iterations);
+    d.eig_safety = data.eig_safety;
+    d.coarse_max_dofs = data.coarse_max_dofs;
+    check(mfx_mg_create(&d, &mg_));
+    mfx_mg_level_info info{};
+    check(mfx_mg_get_level(mg_, 0, &info));
+    fine_.push_back(LaplaceOperator<double>::view(info.op));
+  }
+  PreconditionMG(const PreconditionMG &) = delete;
+  PreconditionMG &operator=(const PreconditionMG &) = delete;
+  ~PreconditionMG() {
+    fine_.clear();
+    if (mg_) mfx_mg_destroy(mg_);
+  }
+  const LaplaceOperator<double> &get_operator() const { return fine_.front(); }
+  unsigned n_levels() const { return static_cast<unsigned>(mfx_mg_n_levels(mg_)); }
+  mfx_mg_level_info level(unsigned l) const {
+    mfx_mg_level_info info{};
+    check(mfx_mg_get_level(mg_, static_cast<int>(l), &info));
+    return info;
+  }
+  // dst = V-cycle(src)
+  void vmult(Vector<double> &dst, const Vector<double> &src) const {
+    check(mfx_mg_vmult(mg_, dst.get_values(), src.get_values(), nullptr));
+  }
+  mfx_mg *handle() const { return mg_; }
+
+ private:
+  mfx_mg *mg_ = nullptr;
+  std::vector<LaplaceOperator<double>> fine_;
 };
 
 class SolverControl {
@@ -254,6 +326,20 @@ class SolverCG {
   void solve(const MatrixType &A, VectorType &x, const VectorType &b,
              const DiagonalMatrix<VectorType> &P) {
     run(A, x, b, P.get_vector().get_values());
+  }
+  // FP64 CG preconditioned by one multigrid V-cycle per step
+  template <typename MatrixType>
+  void solve(const MatrixType &A, VectorType &x, const VectorType &b, const PreconditionMG &P) {
+    if (A.handle() != P.get_operator().handle())
+      throw Error(MFX_ERR_INVALID_ARG, "A must be the multigrid's fine operator (get_operator())");
+    mfx_cg_result r{};
+    const mfx_status s = mfx_mg_solve(P.handle(), x.get_values(), b.get_values(),
+                                      static_cast<int>(control_.max_steps()),
+                                      control_.tolerance(), control_.reduction(), &r, nullptr);
+    control_.record(r);
+    if (s == MFX_ERR_NOT_CONVERGED || s == MFX_ERR_BREAKDOWN)
+      throw SolverControl::NoConvergence(control_.last_step(), control_.last_value());
+    check(s);
   }
 
  private:

@@ -49,3 +49,17 @@ def test_driver_bp5_runs(gpu):
     text = run("--bp5", 4, 8, 1)
     m = re.search(r"RESULT bp5 p=4 cells=512 dofs=35937 .* cg_its=(\d+)", text)
     assert m and int(m.group(1)) > 10
+
+
+def test_driver_mixed_precision_multigrid(gpu):
+    """PreconditionMG through the C++ API: FP64 CG to 1e-10 with the FP32
+    V-cycle takes a handful of steps where Jacobi-CG takes hundreds, and the
+    two solutions agree to the Jacobi solve's 1e-8 tolerance."""
+    text = run("--mg", 4, 16)
+    m = re.search(r"RESULT mg p=4 cells=4096 dofs=274625 .* mg_its=(\d+) .* jacobi_its=(\d+) "
+                  r".* rel_diff=(\S+)", text)
+    assert m, text
+    mg_its, j_its, diff = int(m.group(1)), int(m.group(2)), float(m.group(3))
+    assert mg_its <= 12 and j_its >= 10 * mg_its
+    assert diff <= 1e-6
+    assert "levels: Q4/16^3 Q2/16^3 Q1/16^3 Q1/8^3" in text
