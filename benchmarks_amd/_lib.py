@@ -66,6 +66,18 @@ class MgLevelInfo(C.Structure):
                 ("op", C.c_void_p)]
 
 
+class BatchInstance(C.Structure):
+    _fields_ = [("degree", C.c_int), ("n_cells", C.c_uint32 * 3), ("lo", C.c_double * 3),
+                ("hi", C.c_double * 3), ("deform", C.c_double)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("degree", C.c_int), ("n_cells", C.c_uint32), ("n_variants", C.c_int),
+                ("variant", C.c_int), ("default_variant", C.c_int),
+                ("cells_per_workgroup", C.c_int * 8), ("prefetch", C.c_int * 8),
+                ("ms", C.c_double * 8)]
+
+
 _P = C.c_void_p
 _SIGS = {
     "mfx_last_error": (C.c_char_p, []),
@@ -108,6 +120,18 @@ _SIGS = {
     "mfx_transfer_restrict_f64": (C.c_int, [_P, _P, _P, _P, _P]),
     "mfx_chebyshev_f32": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double, C.c_int,
                                     _P]),
+    "mfx_batch_create": (C.c_int, [C.POINTER(BatchInstance), C.c_int, C.c_int,
+                                   C.POINTER(C.c_void_p)]),
+    "mfx_batch_destroy": (C.c_int, [_P]),
+    "mfx_batch_n_instances": (C.c_int, [_P]),
+    "mfx_batch_n_dofs": (C.c_uint64, [_P]),
+    "mfx_batch_offsets": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "mfx_batch_instance_operator": (_P, [_P, C.c_int]),
+    "mfx_batch_get_tuning": (C.c_int, [_P, C.c_int, C.POINTER(BatchTuning)]),
+    "mfx_batch_vmult": (C.c_int, [_P, _P, _P, _P]),
+    "mfx_batch_inverse_diagonal": (C.c_int, [_P, _P, _P]),
+    "mfx_batch_cg_solve": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double, C.c_int,
+                                     C.POINTER(CgResult), _P]),
 }
 
 _lib = None

@@ -67,14 +67,17 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 // ---------------------------------------------------------------------------
 // GLL collocation (BK5): NM == NQ, B = I.
 // ---------------------------------------------------------------------------
-template <typename T, int NQ, bool GATHER, bool DOT>
-__global__ void __launch_bounds__(CollocTile<NQ>::THREADS)
+// EPB cells per workgroup; PREF: G columns loaded into registers before the
+// first barrier.  cell_inst / inst_done (optional): cells whose instance is
+// flagged done are skipped (batched solves).
+template <typename T, int NQ, int EPB, bool PREF, bool GATHER, bool DOT>
+__global__ void __launch_bounds__(EPB * NQ * NQ)
 k_apply_colloc(const T *__restrict__ gD, const T *__restrict__ G,
                const T *__restrict__ src, T *__restrict__ dst,
                const uint32_t *__restrict__ map, uint32_t n_cells,
-               double *__restrict__ partials, const int *__restrict__ skip) {
-  constexpr int EPB = CollocTile<NQ>::EPB, THREADS = CollocTile<NQ>::THREADS;
-  constexpr bool PREF = CollocTile<NQ>::PREFETCH;
+               double *__restrict__ partials, const int *__restrict__ skip,
+               const uint16_t *__restrict__ cell_inst, const int *__restrict__ inst_done) {
+  constexpr int THREADS = EPB * NQ * NQ;
   constexpr int NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
   __shared__ T sD[NQ * NQ];
   __shared__ T su[EPB][NQ3];
@@ -88,7 +91,8 @@ k_apply_colloc(const T *__restrict__ gD, const T *__restrict__ G,
   const int col = tid - el * NQ2;
   const int qx = col % NQ, qy = col / NQ;
   const uint32_t e = blockIdx.x * EPB + el;
-  const bool active = e < n_cells;
+  bool active = e < n_cells;
+  if (active && inst_done) active = inst_done[cell_inst[e]] == 0;
 
   for (int i = tid; i < NQ * NQ; i += THREADS) sD[i] = gD[i];
 
@@ -363,11 +367,15 @@ void launch_t(const T *B, const T *D, const T *G, const T *src, T *dst,
     const dim3 block(CollocTile<NQ>::THREADS);
     if (partials) {
       if constexpr (std::is_same<T, double>::value && GATHER)
-        hipLaunchKernelGGL((k_apply_colloc<T, NQ, GATHER, true>), grid, block, 0, s,
-                           D, G, src, dst, map, n_cells, partials, skip);
+        hipLaunchKernelGGL((k_apply_colloc<T, NQ, CollocTile<NQ>::EPB, CollocTile<NQ>::PREFETCH,
+                                           GATHER, true>),
+                           grid, block, 0, s, D, G, src, dst, map, n_cells, partials, skip,
+                           nullptr, nullptr);
     } else {
-      hipLaunchKernelGGL((k_apply_colloc<T, NQ, GATHER, false>), grid, block, 0, s,
-                         D, G, src, dst, map, n_cells, nullptr, nullptr);
+      hipLaunchKernelGGL((k_apply_colloc<T, NQ, CollocTile<NQ>::EPB, CollocTile<NQ>::PREFETCH,
+                                         GATHER, false>),
+                         grid, block, 0, s, D, G, src, dst, map, n_cells, nullptr, nullptr,
+                         nullptr, nullptr);
     }
   } else {
     const dim3 block(Tile<NQ>::THREADS);
@@ -414,7 +422,69 @@ void dispatch(int p, int nq, bool colloc, const T *B, const T *D, const T *G,
                   std::to_string(nq) + ")"};
 }
 
+// Tile variants of the collocation kernel the autotuner chooses from: three
+// cell counts per workgroup, each with and without the G prefetch.
+template <int NQ, int EPB, bool PREF>
+void launch_cv(const double *D, const double *G, const double *src, double *dst,
+               const uint32_t *map, uint32_t n_cells, const uint16_t *cell_inst,
+               const int *inst_done, hipStream_t s) {
+  const dim3 grid((n_cells + EPB - 1) / EPB), block(EPB * NQ * NQ);
+  hipLaunchKernelGGL((k_apply_colloc<double, NQ, EPB, PREF, true, false>), grid, block, 0, s, D, G,
+                     src, dst, map, n_cells, nullptr, nullptr, cell_inst, inst_done);
+  MFX_HIP(hipGetLastError());
+}
+template <int NQ, int E0, int E1, int E2>
+void launch_cv_sel(int v, int *epb, int *pref, const double *D, const double *G, const double *src,
+                   double *dst, const uint32_t *map, uint32_t n, const uint16_t *ci,
+                   const int *done, hipStream_t s) {
+  const int e[3] = {E0, E1, E2};
+  MFX_REQUIRE(v >= 0 && v < 6, "tile variant out of range");
+  *epb = e[v / 2];
+  *pref = (v % 2 == 0);
+  if (!src) return;  // description only
+  if (n == 0) return;
+  switch (v) {
+    case 0: return launch_cv<NQ, E0, true>(D, G, src, dst, map, n, ci, done, s);
+    case 1: return launch_cv<NQ, E0, false>(D, G, src, dst, map, n, ci, done, s);
+    case 2: return launch_cv<NQ, E1, true>(D, G, src, dst, map, n, ci, done, s);
+    case 3: return launch_cv<NQ, E1, false>(D, G, src, dst, map, n, ci, done, s);
+    case 4: return launch_cv<NQ, E2, true>(D, G, src, dst, map, n, ci, done, s);
+    default: return launch_cv<NQ, E2, false>(D, G, src, dst, map, n, ci, done, s);
+  }
+}
+
 }  // namespace
+
+int colloc_variant_count(int p) { return (p >= 2 && p <= 6) ? 6 : 0; }
+
+int colloc_default_variant(int p) {
+  // the variant equal to the default tile (CollocTile) of launch_apply
+  switch (p) {
+    case 2: return 2;  // 14 cells, prefetch
+    case 3: return 2;  // 8 cells, prefetch
+    case 4: return 2;  // 5 cells, prefetch
+    case 5: return 2;  // 3 cells, prefetch
+    case 6: return 0;  // 1 cell, prefetch
+    default: return 0;
+  }
+}
+
+void launch_colloc_variant(int p, int v, const double *D, const double *G, const double *src,
+                           double *dst, const uint32_t *map, uint32_t n_cells,
+                           const uint16_t *cell_inst, const int *inst_done, hipStream_t s,
+                           int *epb, int *prefetch) {
+  int e = 0, pf = 0;
+  switch (p) {
+    case 2: launch_cv_sel<3, 7, 14, 28>(v, &e, &pf, D, G, src, dst, map, n_cells, cell_inst, inst_done, s); break;
+    case 3: launch_cv_sel<4, 4, 8, 16>(v, &e, &pf, D, G, src, dst, map, n_cells, cell_inst, inst_done, s); break;
+    case 4: launch_cv_sel<5, 2, 5, 10>(v, &e, &pf, D, G, src, dst, map, n_cells, cell_inst, inst_done, s); break;
+    case 5: launch_cv_sel<6, 1, 3, 7>(v, &e, &pf, D, G, src, dst, map, n_cells, cell_inst, inst_done, s); break;
+    case 6: launch_cv_sel<7, 1, 2, 5>(v, &e, &pf, D, G, src, dst, map, n_cells, cell_inst, inst_done, s); break;
+    default: throw Error{MFX_ERR_UNSUPPORTED, "tile variants exist for degrees 2..6"};
+  }
+  if (epb) *epb = e;
+  if (prefetch) *prefetch = pf;
+}
 
 template <typename T>
 void launch_apply(int p, int nq, bool colloc, const T *B, const T *D, const T *G,

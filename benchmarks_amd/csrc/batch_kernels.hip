@@ -1,0 +1,288 @@
+// batch_kernels.hip -- vector and scalar steps of the batched CG: many
+// independent problems concatenated into one DoF vector, each with its own
+// CG scalars and convergence state.
+//
+// Vector kernels run one workgroup per segment (a contiguous range inside
+// one instance), skip segments of converged instances, and write one partial
+// per segment.  The scalar kernels run one workgroup in total; a thread owns
+// an instance and sums that instance's segment partials in index order, so
+// every reduction is deterministic.
+#include <algorithm>
+
+#include "batch.hpp"
+#include "mfx_internal.hpp"
+
+namespace mfx {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void block_sum2(double &a, double &b) {
+  __shared__ double ra[kBlock / 64], rb[kBlock / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    ra[w] = a;
+    rb[w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) {
+      sa += ra[i];
+      sb += rb[i];
+    }
+    a = sa;
+    b = sb;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_residual(const Seg *__restrict__ segs, double *__restrict__ r, const double *__restrict__ b,
+             const double *__restrict__ t, const double *__restrict__ dinv,
+             double *__restrict__ part_rr, double *__restrict__ part_rz) {
+  const Seg sg = segs[blockIdx.x];
+  double rr = 0.0, rz = 0.0;
+  for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) {
+    const double v = b[i] - t[i];
+    r[i] = v;
+    rr += v * v;
+    rz += v * dinv[i] * v;
+  }
+  block_sum2(rr, rz);
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = rr;
+    part_rz[blockIdx.x] = rz;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_init(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg, uint32_t n_inst,
+         const double *__restrict__ part_rr, const double *__restrict__ part_rz, double abs_tol,
+         double rel_tol, int max_steps, int *__restrict__ flags, int *__restrict__ n_active) {
+  __shared__ int active;
+  if (threadIdx.x == 0) active = 0;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n_inst; i += kBlock) {
+    double rr = 0.0, rz = 0.0;
+    for (uint32_t k = inst_seg[i]; k < inst_seg[i + 1]; ++k) {
+      rr += part_rr[k];
+      rz += part_rz[k];
+    }
+    BatchCg c{};
+    c.initial = sqrt(rr);
+    c.last = c.initial;
+    c.tol = fmax(abs_tol, rel_tol * c.initial);
+    c.gamma = rz;
+    c.max_steps = max_steps;
+    c.first = 1;
+    if (c.initial <= c.tol) {
+      c.done = 1;
+      c.converged = 1;
+    } else if (max_steps <= 0) {
+      c.done = 1;
+    }
+    st[i] = c;
+    flags[i] = c.done;
+    if (!c.done) atomicAdd(&active, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *n_active = active;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_direction(const Seg *__restrict__ segs, const BatchCg *__restrict__ st,
+              const double *__restrict__ r, const double *__restrict__ dinv,
+              double *__restrict__ p, double *__restrict__ ap) {
+  const Seg sg = segs[blockIdx.x];
+  const BatchCg &c = st[sg.inst];
+  if (c.done) return;
+  const double beta = c.first ? 0.0 : c.beta;
+  for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) {
+    p[i] = dinv[i] * r[i] + (beta != 0.0 ? beta * p[i] : 0.0);
+    ap[i] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_pap(const Seg *__restrict__ segs, const BatchCg *__restrict__ st,
+        const double *__restrict__ p, const double *__restrict__ ap, double *__restrict__ part) {
+  const Seg sg = segs[blockIdx.x];
+  if (st[sg.inst].done) return;
+  double s = 0.0, unused = 0.0;
+  for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) s += p[i] * ap[i];
+  block_sum2(s, unused);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_alpha(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg, uint32_t n_inst,
+          const double *__restrict__ part) {
+  for (uint32_t i = threadIdx.x; i < n_inst; i += kBlock) {
+    BatchCg &c = st[i];
+    if (c.done) continue;
+    double pap = 0.0;
+    for (uint32_t k = inst_seg[i]; k < inst_seg[i + 1]; ++k) pap += part[k];
+    if (!(pap > 0.0)) {
+      c.breakdown = 1;
+      c.alpha = 0.0;
+    } else {
+      c.alpha = c.gamma / pap;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_update(const Seg *__restrict__ segs, const BatchCg *__restrict__ st, double *__restrict__ x,
+           double *__restrict__ r, const double *__restrict__ p, const double *__restrict__ ap,
+           const double *__restrict__ dinv, double *__restrict__ part_rr,
+           double *__restrict__ part_rz) {
+  const Seg sg = segs[blockIdx.x];
+  const BatchCg &c = st[sg.inst];
+  if (c.done) return;
+  const double alpha = c.alpha;
+  double rr = 0.0, rz = 0.0;
+  for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) {
+    x[i] += alpha * p[i];
+    const double v = r[i] - alpha * ap[i];
+    r[i] = v;
+    rr += v * v;
+    rz += v * dinv[i] * v;
+  }
+  block_sum2(rr, rz);
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = rr;
+    part_rz[blockIdx.x] = rz;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_check(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg, uint32_t n_inst,
+          const double *__restrict__ part_rr, const double *__restrict__ part_rz,
+          int *__restrict__ flags, int *__restrict__ n_active) {
+  __shared__ int active;
+  if (threadIdx.x == 0) active = 0;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n_inst; i += kBlock) {
+    BatchCg &c = st[i];
+    if (!c.done) {
+      double rr = 0.0, rz = 0.0;
+      for (uint32_t k = inst_seg[i]; k < inst_seg[i + 1]; ++k) {
+        rr += part_rr[k];
+        rz += part_rz[k];
+      }
+      c.steps += 1;
+      c.last = sqrt(rr);
+      if (c.breakdown) {
+        c.done = 1;
+      } else if (c.last <= c.tol) {
+        c.done = 1;
+        c.converged = 1;
+      } else if (c.steps >= c.max_steps) {
+        c.done = 1;
+      } else {
+        c.beta = rz / c.gamma;
+        c.gamma = rz;
+        c.first = 0;
+      }
+    }
+    flags[i] = c.done;
+    if (!c.done) atomicAdd(&active, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *n_active = active;
+}
+
+__global__ void k_offset_map(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                             uint64_t n, uint32_t offset) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = src[i];
+    dst[i] = v == kInvalid ? kInvalid : v + offset;
+  }
+}
+
+__global__ void k_fill_u16(uint16_t *__restrict__ dst, uint16_t v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = v;
+}
+
+inline unsigned grid_for(uint64_t n) {
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + kBlock - 1) / kBlock, 65536));
+}
+
+}  // namespace
+
+void launch_b_residual(const Seg *segs, uint32_t n_segs, double *r, const double *b,
+                       const double *t, const double *dinv, double *part_rr, double *part_rz,
+                       hipStream_t s) {
+  if (!n_segs) return;
+  hipLaunchKernelGGL(k_b_residual, dim3(n_segs), dim3(kBlock), 0, s, segs, r, b, t, dinv,
+                     part_rr, part_rz);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_b_init(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part_rr,
+                   const double *part_rz, double abs_tol, double rel_tol, int max_steps,
+                   int *flags, int *n_active, hipStream_t s) {
+  hipLaunchKernelGGL(k_b_init, dim3(1), dim3(kBlock), 0, s, st, inst_seg, n_inst, part_rr,
+                     part_rz, abs_tol, rel_tol, max_steps, flags, n_active);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_b_direction(const Seg *segs, uint32_t n_segs, const BatchCg *st, const double *r,
+                        const double *dinv, double *p, double *ap, hipStream_t s) {
+  if (!n_segs) return;
+  hipLaunchKernelGGL(k_b_direction, dim3(n_segs), dim3(kBlock), 0, s, segs, st, r, dinv, p, ap);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_b_pap(const Seg *segs, uint32_t n_segs, const BatchCg *st, const double *p,
+                  const double *ap, double *part, hipStream_t s) {
+  if (!n_segs) return;
+  hipLaunchKernelGGL(k_b_pap, dim3(n_segs), dim3(kBlock), 0, s, segs, st, p, ap, part);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_b_alpha(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_b_alpha, dim3(1), dim3(kBlock), 0, s, st, inst_seg, n_inst, part);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_b_update(const Seg *segs, uint32_t n_segs, const BatchCg *st, double *x, double *r,
+                     const double *p, const double *ap, const double *dinv, double *part_rr,
+                     double *part_rz, hipStream_t s) {
+  if (!n_segs) return;
+  hipLaunchKernelGGL(k_b_update, dim3(n_segs), dim3(kBlock), 0, s, segs, st, x, r, p, ap, dinv,
+                     part_rr, part_rz);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_b_check(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part_rr,
+                    const double *part_rz, int *flags, int *n_active, hipStream_t s) {
+  hipLaunchKernelGGL(k_b_check, dim3(1), dim3(kBlock), 0, s, st, inst_seg, n_inst, part_rr,
+                     part_rz, flags, n_active);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_offset_map(const uint32_t *src, uint32_t *dst, uint64_t n, uint32_t offset,
+                       hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_offset_map, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n, offset);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_fill_u16(uint16_t *dst, uint16_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_u16, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, v, n);
+  MFX_HIP(hipGetLastError());
+}
+
+}  // namespace mfx

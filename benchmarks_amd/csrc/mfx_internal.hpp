@@ -44,6 +44,17 @@ template <typename T>
 void launch_evec(int p, int nq, bool collocation, const T *B, const T *D,
                  const T *G, const T *in, T *out, uint32_t n_cells,
                  hipStream_t s);
+// Tile variants of the FP64 collocation apply (degrees 2..6) for the
+// autotuner: variant v in [0, colloc_variant_count(p)); dst += A src over the
+// cells; with inst_done set, cells of instances flagged done are skipped.
+// epb / prefetch (optional) receive the variant's tile; src == nullptr only
+// describes the variant.
+int colloc_variant_count(int p);
+int colloc_default_variant(int p);
+void launch_colloc_variant(int p, int v, const double *D, const double *G, const double *src,
+                           double *dst, const uint32_t *map, uint32_t n_cells,
+                           const uint16_t *cell_inst, const int *inst_done, hipStream_t s,
+                           int *epb = nullptr, int *prefetch = nullptr);
 // number of partial sums written by launch_apply(..., dot_partials)
 uint32_t apply_n_blocks(int p, int nq, bool collocation, uint32_t n_cells);
 

@@ -354,6 +354,117 @@ class SolverCG:
         check(st)
 
 
+class BatchedLaplace:
+    """Many independent BP5 problems (GLL collocation, degree 2..6, deformed
+    boxes) on one concatenated DoF vector (mfx_batch_*): instance i owns
+    [offsets[i], offsets[i+1]).  No counterpart in the reference, which runs
+    one problem per process.  instances: iterable of (degree, n_cells) or
+    (degree, n_cells, deform)."""
+
+    def __init__(self, instances, autotune: bool = True, deform: float = 0.05):
+        inst = list(instances)
+        arr = (_lib.BatchInstance * len(inst))()
+        for i, it in enumerate(inst):
+            p, n = it[0], it[1]
+            d = it[2] if len(it) > 2 else deform
+            n = (n,) * 3 if isinstance(n, int) else n
+            arr[i].degree = p
+            for k in range(3):
+                arr[i].n_cells[k], arr[i].lo[k], arr[i].hi[k] = n[k], 0.0, 1.0
+            arr[i].deform = d
+        self._h = None
+        h = C.c_void_p()
+        check(lib().mfx_batch_create(arr, len(inst), int(autotune), C.byref(h)))
+        self._h = h
+        self.n_instances = len(inst)
+        off = (C.c_uint64 * (len(inst) + 1))()
+        check(lib().mfx_batch_offsets(self._h, off))
+        self.offsets = [int(v) for v in off]
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and lib is not None:
+            try:
+                lib().mfx_batch_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def m(self) -> int:
+        return int(lib().mfx_batch_n_dofs(self._h))
+
+    def initialize_dof_vector(self) -> torch.Tensor:
+        return torch.zeros(self.m(), dtype=torch.float64, device="cuda")
+
+    def instance_operator(self, i: int) -> LaplaceOperator:
+        h = lib().mfx_batch_instance_operator(self._h, i)
+        return LaplaceOperator(0, 0, _handle=C.c_void_p(h), _owner=self)
+
+    def vmult(self, dst: torch.Tensor, src: torch.Tensor) -> None:
+        check(lib().mfx_batch_vmult(self._h, _ptr(dst), _ptr(src), _stream()))
+
+    def compute_inverse_diagonal(self) -> torch.Tensor:
+        d = self.initialize_dof_vector()
+        check(lib().mfx_batch_inverse_diagonal(self._h, _ptr(d), _stream()))
+        return d
+
+    def zero_constrained(self, v: torch.Tensor) -> None:
+        for i in range(self.n_instances):
+            self.instance_operator(i).zero_constrained(v[self.offsets[i]:self.offsets[i + 1]])
+
+    def tuning(self, degree: int) -> dict:
+        t = _lib.BatchTuning()
+        check(lib().mfx_batch_get_tuning(self._h, degree, C.byref(t)))
+        nv = t.n_variants
+        return {"degree": t.degree, "n_cells": t.n_cells, "variant": t.variant,
+                "default_variant": t.default_variant,
+                "cells_per_workgroup": list(t.cells_per_workgroup)[:nv],
+                "prefetch": list(t.prefetch)[:nv], "ms": list(t.ms)[:nv]}
+
+    def solve(self, x: torch.Tensor, b: torch.Tensor, inverse_diagonal=None, max_steps=100000,
+              abs_tol=1e-300, rel_tol=1e-8, use_graph=True):
+        """Independent Jacobi-CG (identity if inverse_diagonal is None) per
+        instance; returns one ReductionControl per instance.  Raises
+        NoConvergence if any instance fails."""
+        res = (_lib.CgResult * self.n_instances)()
+        inv = None if inverse_diagonal is None else _ptr(inverse_diagonal)
+        st = lib().mfx_batch_cg_solve(self._h, _ptr(x), _ptr(b), inv, max_steps, abs_tol, rel_tol,
+                                      int(use_graph), res, _stream())
+        out = []
+        for r in res:
+            c = ReductionControl(max_steps, abs_tol, rel_tol)
+            c._last_step, c._initial, c._last, c.seconds = (r.steps, r.initial_value,
+                                                            r.last_value, r.seconds)
+            out.append(c)
+        if st == 4:
+            raise NoConvergence(max(r.steps for r in res), max(r.last_value for r in res))
+        check(st)
+        return out
+
+
+def batch_instances(count: int = 64, seed: int = 2, n_min: int = 4, n_max: int = 32,
+                    degrees=(2, 3, 4, 5, 6)):
+    """Instance list of the batched-solve configuration: cells per direction
+    log-uniform in [n_min, n_max] (rounded), degree uniform over `degrees`,
+    from the SplitMix64 stream of `seed` (u = (z >> 11) 2^-53 in [0,1))."""
+    import math
+    M = (1 << 64) - 1
+    out, k = [], 0
+
+    def u01():
+        nonlocal k
+        k += 1
+        z = (seed + k * 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        return (z >> 11) * 2.0 ** -53
+    for _ in range(count):
+        n = int(round(math.exp(math.log(n_min) + u01() * (math.log(n_max) - math.log(n_min)))))
+        p = degrees[min(int(u01() * len(degrees)), len(degrees) - 1)]
+        out.append((p, n))
+    return out
+
+
 # --- kernel-level plug point -------------------------------------------
 def bk5_apply(degree, D, G, src, dst, map_, n_cells):
     """dst += sum_cells P^T D^T G D P src (GLL collocation), no zeroing."""

@@ -307,6 +307,56 @@ mfx_status mfx_chebyshev_f32(mfx_op *op, const float *inv_diag, float *x, const 
                              int degree, double lambda_min, double lambda_max,
                              int x_is_zero, void *stream);
 
+/* ----------------------------------------------------------------------
+ * Batched solves.  No counterpart in the reference, which runs one problem
+ * per process (CEED_bp/src/bp3.cc:423-534).  Many independent BP5 problems
+ * (GLL collocation q = p+1, degree 2..6, stored geometry) are concatenated
+ * into one DoF vector, instance i occupying [offsets[i], offsets[i+1]).
+ * mfx_batch_vmult applies every instance's operator (vmult semantics per
+ * instance); mfx_batch_cg_solve runs an independent Jacobi-CG per instance
+ * (mfx_cg_solve's stopping rule and result fields, per instance) as a
+ * hipGraph of CG steps replayed until all have converged.
+ * ---------------------------------------------------------------------- */
+typedef struct mfx_batch mfx_batch;
+
+typedef struct {
+  int degree;           /* 2..6 */
+  uint32_t n_cells[3];
+  double lo[3], hi[3];
+  double deform;
+} mfx_batch_instance;
+
+typedef struct {
+  int degree;
+  uint32_t n_cells;            /* cells of this degree over all instances */
+  int n_variants;              /* tile variants of the collocation kernel */
+  int variant;                 /* the one in use */
+  int default_variant;         /* the library's default tile */
+  int cells_per_workgroup[8];  /* per variant */
+  int prefetch[8];             /* per variant: metric terms loaded ahead */
+  double ms[8];                /* per variant: measured apply time (0 if not tuned) */
+} mfx_batch_tuning;
+
+/* autotune != 0: time every tile variant per degree on the batch's cells and
+ * keep the fastest; otherwise the default tile */
+mfx_status mfx_batch_create(const mfx_batch_instance *instances, int n_instances,
+                            int autotune, mfx_batch **out);
+mfx_status mfx_batch_destroy(mfx_batch *batch);
+int mfx_batch_n_instances(const mfx_batch *batch);
+uint64_t mfx_batch_n_dofs(const mfx_batch *batch);
+mfx_status mfx_batch_offsets(const mfx_batch *batch, uint64_t *offsets /* n + 1 */);
+/* the instance's own operator (owned by the batch) */
+mfx_op *mfx_batch_instance_operator(mfx_batch *batch, int instance);
+mfx_status mfx_batch_get_tuning(const mfx_batch *batch, int degree, mfx_batch_tuning *tuning);
+mfx_status mfx_batch_vmult(mfx_batch *batch, double *dst, const double *src, void *stream);
+mfx_status mfx_batch_inverse_diagonal(mfx_batch *batch, double *inv, void *stream);
+/* inv_diag NULL: identity preconditioner.  results: one per instance.
+ * use_graph = 0 issues the same launches directly. */
+mfx_status mfx_batch_cg_solve(mfx_batch *batch, double *x, const double *b,
+                              const double *inv_diag, int max_steps, double abs_tol,
+                              double rel_tol, int use_graph, mfx_cg_result *results,
+                              void *stream);
+
 #ifdef __cplusplus
 }
 #endif

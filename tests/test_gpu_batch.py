@@ -1,0 +1,170 @@
+"""Batched solves (mfx_batch_*): many BP5 problems of uneven size and degree
+on one concatenated vector.  Parity: the batched operator equals every
+instance's own operator (<= 1e-13 relative; only the atomic summation order
+differs), per-instance Jacobi-CG takes the oracle's number of steps (the
+knife-edge rule of test_gpu_cg), the hipGraph path equals the direct one, and
+the autotuner keeps the fastest measured tile."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from test_gpu_cg import assert_same_steps
+
+pytestmark = pytest.mark.gpu
+
+SMALL = [(2, (3, 4, 2)), (5, 2), (3, (4, 3, 5)), (6, 2), (4, 3), (2, 5), (3, 1)]
+
+
+def _batch(instances, autotune=False):
+    from benchmarks_amd.laplace import BatchedLaplace
+    return BatchedLaplace(instances, autotune=autotune)
+
+
+def _rhs(B, seed=2):
+    from benchmarks_amd.laplace import random_vector
+    b = random_vector(B.m(), seed)
+    B.zero_constrained(b)
+    return b
+
+
+def test_batch_layout(gpu):
+    B = _batch(SMALL)
+    sizes = [np.prod([(n if isinstance(n, int) else n[k]) * p + 1 for k in range(3)])
+             for p, n in SMALL]
+    assert B.offsets == [0] + list(np.cumsum(sizes))
+    assert B.m() == sum(sizes)
+
+
+def test_batch_vmult_equals_instance_operators(gpu):
+    import torch
+    from benchmarks_amd.laplace import random_vector
+    B = _batch(SMALL)
+    x = random_vector(B.m(), 9)
+    y = B.initialize_dof_vector()
+    B.vmult(y, x)
+    for i in range(len(SMALL)):
+        a, b = B.offsets[i], B.offsets[i + 1]
+        op = B.instance_operator(i)
+        yi = op.initialize_dof_vector()
+        op.vmult(yi, x[a:b].clone())
+        assert (torch.linalg.norm(y[a:b] - yi) / torch.linalg.norm(yi)).item() <= 1e-13, i
+
+
+def test_batch_vmult_matches_oracle(gpu):
+    B = _batch(SMALL)
+    from benchmarks_amd.laplace import random_vector
+    x = random_vector(B.m(), 4)
+    y = B.initialize_dof_vector()
+    B.vmult(y, x)
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    for i in (0, 2, 4):
+        p, n = SMALL[i]
+        P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
+        a, b = B.offsets[i], B.offsets[i + 1]
+        ref = P.vmult(xh[a:b])
+        assert np.linalg.norm(yh[a:b] - ref) <= 1e-12 * np.linalg.norm(ref), i
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_cg_matches_oracle(gpu, use_graph):
+    B = _batch(SMALL)
+    b = _rhs(B)
+    inv = B.compute_inverse_diagonal()
+    x = B.initialize_dof_vector()
+    ctls = B.solve(x, b, inv, rel_tol=1e-8, use_graph=use_graph)
+    bh, xh = b.cpu().numpy(), x.cpu().numpy()
+    for i, (p, n) in enumerate(SMALL):
+        a, e = B.offsets[i], B.offsets[i + 1]
+        P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
+        xo, r, hist = P.cg(bh[a:e], inv_diag=P.inverse_diagonal(), rel_tol=1e-8, abs_tol=1e-300)
+        assert r.converged
+        assert_same_steps(ctls[i].last_step(), r.steps, hist, 1e-8)
+        assert ctls[i].last_value() <= 1e-8 * ctls[i].initial_value()
+        assert np.linalg.norm(xh[a:e] - xo) <= 1e-6 * np.linalg.norm(xo), i
+
+
+def test_batch_graph_equals_direct(gpu):
+    import torch
+    B = _batch(SMALL)
+    b = _rhs(B, 5)
+    inv = B.compute_inverse_diagonal()
+    x1, x2 = B.initialize_dof_vector(), B.initialize_dof_vector()
+    c1 = B.solve(x1, b, inv, rel_tol=1e-9, use_graph=True)
+    c2 = B.solve(x2, b, inv, rel_tol=1e-9, use_graph=False)
+    for a, c in zip(c1, c2):
+        assert abs(a.last_step() - c.last_step()) <= 1
+    assert (torch.linalg.norm(x1 - x2) / torch.linalg.norm(x2)).item() <= 1e-8
+
+
+def test_batch_cg_matches_single_solves(gpu):
+    """The batched CG against mfx_cg_solve on each instance's own operator."""
+    import torch
+    from benchmarks_amd.laplace import DiagonalMatrix, ReductionControl, SolverCG
+    inst = [(4, 10), (6, 6), (2, 16), (3, (12, 8, 10)), (5, 7)]
+    B = _batch(inst, autotune=True)
+    b = _rhs(B, 3)
+    inv = B.compute_inverse_diagonal()
+    x = B.initialize_dof_vector()
+    ctls = B.solve(x, b, inv, rel_tol=1e-8)
+    for i in range(len(inst)):
+        a, e = B.offsets[i], B.offsets[i + 1]
+        op = B.instance_operator(i)
+        xi = op.initialize_dof_vector()
+        c = ReductionControl(100000, 1e-300, 1e-8)
+        SolverCG(c).solve(op, xi, b[a:e].clone(), DiagonalMatrix(inv[a:e].clone()))
+        assert abs(c.last_step() - ctls[i].last_step()) <= 1, (i, c.last_step(), ctls[i].last_step())
+        assert (torch.linalg.norm(x[a:e] - xi) / torch.linalg.norm(xi)).item() <= 1e-6
+
+
+def test_batch_converged_instance_is_left_alone(gpu):
+    """An instance with a zero right-hand side is done before the first step;
+    its solution stays zero while the others iterate."""
+    B = _batch([(3, 4), (2, 6), (4, 3)])
+    b = _rhs(B)
+    a, e = B.offsets[1], B.offsets[2]
+    b[a:e] = 0
+    x = B.initialize_dof_vector()
+    ctls = B.solve(x, b, B.compute_inverse_diagonal(), abs_tol=1e-30, rel_tol=1e-8)
+    assert ctls[1].last_step() == 0 and x[a:e].abs().max().item() == 0.0
+    assert ctls[0].last_step() > 5 and ctls[2].last_step() > 5
+
+
+def test_autotuner_keeps_fastest_variant(gpu):
+    import torch
+    from benchmarks_amd.laplace import random_vector
+    inst = [(2, 12), (3, 10), (4, 8), (5, 7), (6, 6)]
+    B = _batch(inst, autotune=True)
+    U = _batch(inst, autotune=False)
+    for p in range(2, 7):
+        t = B.tuning(p)
+        assert len(t["ms"]) == 6 and min(t["ms"]) > 0
+        assert t["ms"][t["variant"]] == min(t["ms"])
+        u = U.tuning(p)
+        assert u["variant"] == u["default_variant"] and max(u["ms"]) == 0
+    x = random_vector(B.m(), 1)
+    y1, y2 = B.initialize_dof_vector(), U.initialize_dof_vector()
+    B.vmult(y1, x)
+    U.vmult(y2, x)
+    assert (torch.linalg.norm(y1 - y2) / torch.linalg.norm(y2)).item() <= 1e-13
+
+
+def test_batch_rejects_bad_input(gpu):
+    from benchmarks_amd._lib import MfxError
+    with pytest.raises(MfxError):
+        _batch([(7, 3)])
+    with pytest.raises(MfxError):
+        _batch([])
+
+
+def test_batched_configuration_converges(gpu):
+    """The 64-instance configuration (cells per direction log-uniform in
+    [4, 32], degrees 2..6, seed 2): every instance reaches 1e-8."""
+    from benchmarks_amd.laplace import batch_instances
+    inst = batch_instances(64, 2)
+    B = _batch(inst, autotune=True)
+    b = _rhs(B, 2)
+    x = B.initialize_dof_vector()
+    ctls = B.solve(x, b, B.compute_inverse_diagonal(), rel_tol=1e-8)
+    assert all(c.last_value() <= 1e-8 * c.initial_value() for c in ctls)
+    steps = [c.last_step() for c in ctls]
+    assert min(steps) > 0
