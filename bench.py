@@ -17,7 +17,9 @@ algorithmic bytes and FLOPs, Jacobi-CG and mixed-precision multigrid CG
 time-to-1e-8 relative residual on the same operator and right-hand side, and
 the mixed-precision BP3 solve (Q4, Gauss q=6, deformed 128^3 cells, 135M
 DoFs, manufactured-solution right-hand side, FP64 CG to 1e-10 with an FP32
-p/h-multigrid V-cycle) with its error against an FP64 solution.
+p/h-multigrid V-cycle) with its error against an FP64 solution, and the
+batched solve of 64 BP5 problems of uneven size and degree (per-instance
+Jacobi-CG in one hipGraph-replayed batch) against solving them one by one.
 """
 import argparse
 import ctypes as C
@@ -248,6 +250,71 @@ def bench_c3(n=128, p=4, nq=6):
     return out
 
 
+def bench_batch(count=64, seed=2):
+    """Batched solves: `count` BP5 problems (cells per direction log-uniform
+    in [4, 32], degree uniform in 2..6, SplitMix64 seed 2), per-instance
+    uniform[-1,1] right-hand side, per-instance Jacobi-CG to 1e-8, all in one
+    batch (hipGraph of CG steps, autotuned tile per degree), next to the same
+    instances solved one after another with mfx_cg_solve."""
+    import torch
+
+    from benchmarks_amd.laplace import (BatchedLaplace, DiagonalMatrix, ReductionControl,
+                                        SolverCG, batch_instances, random_vector)
+    inst = batch_instances(count, seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B = BatchedLaplace(inst, autotune=True)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    b = random_vector(B.m(), seed)
+    B.zero_constrained(b)
+    inv = B.compute_inverse_diagonal()
+    x = B.initialize_dof_vector()
+    B.solve(x, b, inv, rel_tol=1e-2)  # warm-up (kernel loading)
+    res = {}
+    for graph in (True, False):
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctls = B.solve(x, b, inv, rel_tol=1e-8, use_graph=graph)
+        torch.cuda.synchronize()
+        res["graph" if graph else "direct"] = (time.perf_counter() - t0, ctls)
+    t_seq, steps_seq = 0.0, []
+    for i in range(len(inst)):
+        a, e = B.offsets[i], B.offsets[i + 1]
+        op = B.instance_operator(i)
+        bi, ii = b[a:e].clone(), inv[a:e].clone()
+        xi = op.initialize_dof_vector()
+        c = ReductionControl(100000, 1e-300, 1e-8)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        SolverCG(c).solve(op, xi, bi, DiagonalMatrix(ii))
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t0
+        steps_seq.append(c.last_step())
+    ctls = res["graph"][1]
+    tuning = {}
+    for p in sorted({p for p, _ in inst}):
+        t = B.tuning(p)
+        v, d = t["variant"], t["default_variant"]
+        tuning[p] = {"cells": t["n_cells"], "chosen_cells_per_wg": t["cells_per_workgroup"][v],
+                     "chosen_prefetch": t["prefetch"][v], "ms_chosen": t["ms"][v],
+                     "ms_default": t["ms"][d]}
+    out = {"workload": f"{count} BP5 instances, cells/dir log-uniform 4..32, degree 2..6, "
+                       f"seed {seed}, Jacobi-CG to 1e-8 each",
+           "n_dofs": B.m(), "n_cells": sum(n ** 3 for _, n in inst),
+           "setup_s": t_setup, "batch_graph_s": res["graph"][0], "batch_direct_s": res["direct"][0],
+           "sequential_s": t_seq, "speedup_vs_sequential": t_seq / res["graph"][0],
+           "steps_max": max(c.last_step() for c in ctls),
+           "steps_total": sum(c.last_step() for c in ctls),
+           "steps_differing_from_sequential": sum(1 for c, s in zip(ctls, steps_seq)
+                                                  if c.last_step() != s),
+           "autotune": tuning}
+    del B, b, inv, x
+    torch.cuda.empty_cache()
+    return out
+
+
 def stream_copy_GBs(n=1 << 28, reps=10):
     """Measured HBM ceiling: device-to-device copy of 2 GiB (read + write bytes)."""
     import torch
@@ -341,6 +408,7 @@ def main():
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 135M-DoF mixed-precision solve")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched solve")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the multi-GPU path (slab decomposition with halo "
@@ -402,6 +470,8 @@ def main():
         out["roofline"]["traffic"] = traffic
     if not args.no_cg and not args.no_c3:
         out["mixed_precision_bp3"] = bench_c3()
+    if not args.no_cg and not args.no_batch:
+        out["batched_solves"] = bench_batch()
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))

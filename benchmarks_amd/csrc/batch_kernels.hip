@@ -4,9 +4,9 @@
 //
 // Vector kernels run one workgroup per segment (a contiguous range inside
 // one instance), skip segments of converged instances, and write one partial
-// per segment.  The scalar kernels run one workgroup in total; a thread owns
-// an instance and sums that instance's segment partials in index order, so
-// every reduction is deterministic.
+// per segment.  The scalar kernels run one workgroup per instance, which sums
+// that instance's segment partials in a fixed order, so every reduction is
+// deterministic.
 #include <algorithm>
 
 #include "batch.hpp"
@@ -61,19 +61,29 @@ k_b_residual(const Seg *__restrict__ segs, double *__restrict__ r, const double 
   }
 }
 
+// Sums of an instance's segment partials: one workgroup per instance,
+// threads stride over the segments, fixed-order tree within the workgroup.
+__device__ __forceinline__ void instance_sums(const uint32_t *__restrict__ inst_seg, uint32_t i,
+                                              const double *__restrict__ pa,
+                                              const double *__restrict__ pb, double &a,
+                                              double &b) {
+  a = 0.0;
+  b = 0.0;
+  for (uint32_t k = inst_seg[i] + threadIdx.x; k < inst_seg[i + 1]; k += kBlock) {
+    a += pa[k];
+    if (pb) b += pb[k];
+  }
+  block_sum2(a, b);  // valid in thread 0
+}
+
 __global__ void __launch_bounds__(kBlock)
-k_b_init(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg, uint32_t n_inst,
+k_b_init(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg,
          const double *__restrict__ part_rr, const double *__restrict__ part_rz, double abs_tol,
-         double rel_tol, int max_steps, int *__restrict__ flags, int *__restrict__ n_active) {
-  __shared__ int active;
-  if (threadIdx.x == 0) active = 0;
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n_inst; i += kBlock) {
-    double rr = 0.0, rz = 0.0;
-    for (uint32_t k = inst_seg[i]; k < inst_seg[i + 1]; ++k) {
-      rr += part_rr[k];
-      rz += part_rz[k];
-    }
+         double rel_tol, int max_steps, int *__restrict__ flags) {
+  const uint32_t i = blockIdx.x;
+  double rr, rz;
+  instance_sums(inst_seg, i, part_rr, part_rz, rr, rz);
+  if (threadIdx.x == 0) {
     BatchCg c{};
     c.initial = sqrt(rr);
     c.last = c.initial;
@@ -89,10 +99,16 @@ k_b_init(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg, uint32
     }
     st[i] = c;
     flags[i] = c.done;
-    if (!c.done) atomicAdd(&active, 1);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) *n_active = active;
+}
+
+// number of instances still iterating
+__global__ void __launch_bounds__(kBlock)
+k_b_count(const int *__restrict__ flags, uint32_t n_inst, int *__restrict__ n_active) {
+  double a = 0.0, unused = 0.0;
+  for (uint32_t i = threadIdx.x; i < n_inst; i += kBlock) a += flags[i] ? 0.0 : 1.0;
+  block_sum2(a, unused);
+  if (threadIdx.x == 0) *n_active = (int)a;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -121,13 +137,14 @@ k_b_pap(const Seg *__restrict__ segs, const BatchCg *__restrict__ st,
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_b_alpha(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg, uint32_t n_inst,
+k_b_alpha(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg,
           const double *__restrict__ part) {
-  for (uint32_t i = threadIdx.x; i < n_inst; i += kBlock) {
+  const uint32_t i = blockIdx.x;
+  if (st[i].done) return;
+  double pap, unused;
+  instance_sums(inst_seg, i, part, nullptr, pap, unused);
+  if (threadIdx.x == 0) {
     BatchCg &c = st[i];
-    if (c.done) continue;
-    double pap = 0.0;
-    for (uint32_t k = inst_seg[i]; k < inst_seg[i + 1]; ++k) pap += part[k];
     if (!(pap > 0.0)) {
       c.breakdown = 1;
       c.alpha = 0.0;
@@ -162,40 +179,31 @@ k_b_update(const Seg *__restrict__ segs, const BatchCg *__restrict__ st, double 
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_b_check(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg, uint32_t n_inst,
+k_b_check(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg,
           const double *__restrict__ part_rr, const double *__restrict__ part_rz,
-          int *__restrict__ flags, int *__restrict__ n_active) {
-  __shared__ int active;
-  if (threadIdx.x == 0) active = 0;
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n_inst; i += kBlock) {
+          int *__restrict__ flags) {
+  const uint32_t i = blockIdx.x;
+  if (st[i].done) return;
+  double rr, rz;
+  instance_sums(inst_seg, i, part_rr, part_rz, rr, rz);
+  if (threadIdx.x == 0) {
     BatchCg &c = st[i];
-    if (!c.done) {
-      double rr = 0.0, rz = 0.0;
-      for (uint32_t k = inst_seg[i]; k < inst_seg[i + 1]; ++k) {
-        rr += part_rr[k];
-        rz += part_rz[k];
-      }
-      c.steps += 1;
-      c.last = sqrt(rr);
-      if (c.breakdown) {
-        c.done = 1;
-      } else if (c.last <= c.tol) {
-        c.done = 1;
-        c.converged = 1;
-      } else if (c.steps >= c.max_steps) {
-        c.done = 1;
-      } else {
-        c.beta = rz / c.gamma;
-        c.gamma = rz;
-        c.first = 0;
-      }
+    c.steps += 1;
+    c.last = sqrt(rr);
+    if (c.breakdown) {
+      c.done = 1;
+    } else if (c.last <= c.tol) {
+      c.done = 1;
+      c.converged = 1;
+    } else if (c.steps >= c.max_steps) {
+      c.done = 1;
+    } else {
+      c.beta = rz / c.gamma;
+      c.gamma = rz;
+      c.first = 0;
     }
     flags[i] = c.done;
-    if (!c.done) atomicAdd(&active, 1);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) *n_active = active;
 }
 
 __global__ void k_offset_map(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
@@ -231,8 +239,9 @@ void launch_b_residual(const Seg *segs, uint32_t n_segs, double *r, const double
 void launch_b_init(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part_rr,
                    const double *part_rz, double abs_tol, double rel_tol, int max_steps,
                    int *flags, int *n_active, hipStream_t s) {
-  hipLaunchKernelGGL(k_b_init, dim3(1), dim3(kBlock), 0, s, st, inst_seg, n_inst, part_rr,
-                     part_rz, abs_tol, rel_tol, max_steps, flags, n_active);
+  hipLaunchKernelGGL(k_b_init, dim3(n_inst), dim3(kBlock), 0, s, st, inst_seg, part_rr, part_rz,
+                     abs_tol, rel_tol, max_steps, flags);
+  hipLaunchKernelGGL(k_b_count, dim3(1), dim3(kBlock), 0, s, flags, n_inst, n_active);
   MFX_HIP(hipGetLastError());
 }
 
@@ -252,7 +261,7 @@ void launch_b_pap(const Seg *segs, uint32_t n_segs, const BatchCg *st, const dou
 
 void launch_b_alpha(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part,
                     hipStream_t s) {
-  hipLaunchKernelGGL(k_b_alpha, dim3(1), dim3(kBlock), 0, s, st, inst_seg, n_inst, part);
+  hipLaunchKernelGGL(k_b_alpha, dim3(n_inst), dim3(kBlock), 0, s, st, inst_seg, part);
   MFX_HIP(hipGetLastError());
 }
 
@@ -267,8 +276,9 @@ void launch_b_update(const Seg *segs, uint32_t n_segs, const BatchCg *st, double
 
 void launch_b_check(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part_rr,
                     const double *part_rz, int *flags, int *n_active, hipStream_t s) {
-  hipLaunchKernelGGL(k_b_check, dim3(1), dim3(kBlock), 0, s, st, inst_seg, n_inst, part_rr,
-                     part_rz, flags, n_active);
+  hipLaunchKernelGGL(k_b_check, dim3(n_inst), dim3(kBlock), 0, s, st, inst_seg, part_rr, part_rz,
+                     flags);
+  hipLaunchKernelGGL(k_b_count, dim3(1), dim3(kBlock), 0, s, flags, n_inst, n_active);
   MFX_HIP(hipGetLastError());
 }
 
