@@ -132,6 +132,23 @@ _SIGS = {
     "mfx_batch_inverse_diagonal": (C.c_int, [_P, _P, _P]),
     "mfx_batch_cg_solve": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double, C.c_int,
                                      C.POINTER(CgResult), _P]),
+    "mfx_nn_n_params": (C.c_uint64, []),
+    "mfx_nn_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "mfx_nn_destroy": (C.c_int, [_P]),
+    "mfx_nn_get_params": (C.c_int, [_P, _P]),
+    "mfx_nn_set_params": (C.c_int, [_P, _P]),
+    "mfx_nn_set_output_scale": (C.c_int, [_P, C.c_double]),
+    "mfx_nn_output_scale": (C.c_double, [_P]),
+    "mfx_nn_set_mode": (C.c_int, [_P, C.c_int, C.c_int]),
+    "mfx_nn_forward": (C.c_int, [_P, C.POINTER(C.c_uint32), _P, _P, _P]),
+    "mfx_nn_gradient": (C.c_int, [_P, C.POINTER(C.c_uint32), _P, _P, _P, _P]),
+    "mfx_nn_train_step": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_int, _P, _P, C.c_double,
+                                    C.POINTER(C.c_double), _P]),
+    "mfx_nn_train_energy": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_double,
+                                     C.POINTER(C.c_double), _P]),
+    "mfx_nn_vmult": (C.c_int, [_P, _P, _P, _P, _P]),
+    "mfx_nn_fcg_solve": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double,
+                                   C.POINTER(CgResult), _P]),
 }
 
 _lib = None

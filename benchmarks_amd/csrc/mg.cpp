@@ -166,6 +166,8 @@ double dot(const double *x, const double *y, uint64_t n, DBuf<double> &part, hip
   return h;
 }
 
+}  // namespace
+
 void vmult64(mfx_op &op, double *dst, const double *src, hipStream_t s) {
   launch_fill(dst, 0.0, op.N, s);
   op.apply64(src, dst, s);
@@ -206,6 +208,8 @@ void chebyshev(mfx_op &op, float *x, const float *b, const float *dinv, float *t
   }
 }
 
+namespace {
+
 // largest eigenvalue of a symmetric tridiagonal matrix (cyclic Jacobi on the
 // dense k x k matrix; k is a few tens)
 double tridiag_max_eig(const std::vector<double> &diag, const std::vector<double> &off) {
@@ -243,6 +247,8 @@ double tridiag_max_eig(const std::vector<double> &diag, const std::vector<double
   for (int i = 1; i < k; ++i) m = std::max(m, A[i * k + i]);
   return m;
 }
+
+}  // namespace
 
 // Largest eigenvalue of D^-1 A from `iters` Jacobi-preconditioned CG steps on
 // a random right-hand side: the CG coefficients give the Lanczos tridiagonal
@@ -286,6 +292,8 @@ double estimate_lambda_max(mfx_op &op, const double *inv_diag, int iters, hipStr
   }
   return tridiag_max_eig(dg, od);
 }
+
+namespace {
 
 // In-place inverse of an SPD matrix through its Cholesky factor.
 void spd_inverse(std::vector<double> &A, int n) {

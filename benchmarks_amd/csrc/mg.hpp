@@ -5,7 +5,19 @@
 
 #include <cstdint>
 
+#include "../../include/mfx.h"
+
 namespace mfx {
+
+// operator applications with vmult semantics (zero, apply, identity rows)
+void vmult64(mfx_op &op, double *dst, const double *src, hipStream_t s);
+void vmult32(mfx_op &op, float *dst, const float *src, hipStream_t s);
+// `degree` Chebyshev(Jacobi) steps towards A x = b in FP32 on [lmin, lmax]
+// (t, d: scratch vectors of the operator's size)
+void chebyshev(mfx_op &op, float *x, const float *b, const float *dinv, float *t, float *d,
+               int degree, double lmin, double lmax, bool zero_initial, hipStream_t s);
+// largest eigenvalue of D^-1 A from `iters` CG (Lanczos) steps, FP64
+double estimate_lambda_max(mfx_op &op, const double *inv_diag, int iters, hipStream_t s);
 
 // A 1D transfer matrix in CSR form (device arrays), applied along one axis of
 // a lexicographic (x fastest) node grid.
@@ -43,6 +55,8 @@ void launch_pcg_update(double *x, double *r, const double *p, const double *ap, 
 void launch_scale(double *z, const double *r, const double *dinv, uint64_t n, hipStream_t s);
 // r = b - t
 void launch_residual_f64(double *r, const double *b, const double *t, uint64_t n, hipStream_t s);
+// y += a x
+void launch_axpy_f64(double *y, double a, const double *x, uint64_t n, hipStream_t s);
 // p = z + beta p
 void launch_pcg_direction(double *p, const double *z, double beta, uint64_t n, hipStream_t s);
 

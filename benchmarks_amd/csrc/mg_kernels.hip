@@ -129,6 +129,12 @@ __global__ void k_pcg_direction(double *__restrict__ p, const double *__restrict
     p[i] = z[i] + beta * p[i];
 }
 
+__global__ void k_axpy_f64(double *__restrict__ y, double a, const double *__restrict__ x, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
 __global__ void k_scale(double *__restrict__ z, const double *__restrict__ r,
                         const double *__restrict__ dinv, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -206,6 +212,12 @@ void launch_pcg_update(double *x, double *r, const double *p, const double *ap, 
                        uint64_t n, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_pcg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, x, r, p, ap, alpha, n);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_axpy_f64(double *y, double a, const double *x, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_axpy_f64, dim3(grid_for(n)), dim3(kBlock), 0, s, y, a, x, n);
   MFX_HIP(hipGetLastError());
 }
 

@@ -325,6 +325,12 @@ class SolverCG:
         A._check_vec(x)
         A._check_vec(b)
         res = _lib.CgResult()
+        if isinstance(preconditioner, LearnedPreconditioner):
+            st = lib().mfx_nn_fcg_solve(preconditioner._h, A._h, _ptr(x), _ptr(b),
+                                        self.control.max_steps, self.control.tolerance,
+                                        self.control.reduction, C.byref(res), _stream())
+            self._record(res, st)
+            return
         if isinstance(preconditioner, PreconditionMG):
             if A._h.value != preconditioner.operator()._h.value:
                 raise ValueError("A must be the multigrid's fine operator (PreconditionMG.operator())")
@@ -439,6 +445,95 @@ class BatchedLaplace:
             raise NoConvergence(max(r.steps for r in res), max(r.last_value for r in res))
         check(st)
         return out
+
+
+class LearnedPreconditioner:
+    """The learned preconditioner (mfx_nn_*): a 4-level 3D U-Net on the node
+    grid of a box operator mapping a residual r to z = ||r|| s net(r/||r||),
+    trained on the GPU by the library's own BF16-MFMA forward and FP32
+    backward kernels.  No counterpart in the reference.  With SolverCG it
+    runs flexible CG (mfx_nn_fcg_solve)."""
+
+    def __init__(self, seed: int = 0):
+        self._h = None
+        h = C.c_void_p()
+        check(lib().mfx_nn_create(seed, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and lib is not None:
+            try:
+                lib().mfx_nn_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    @staticmethod
+    def n_params() -> int:
+        return int(lib().mfx_nn_n_params())
+
+    def get_params(self):
+        import numpy as np
+        out = np.zeros(self.n_params(), dtype=np.float32)
+        check(lib().mfx_nn_get_params(self._h, out.ctypes.data))
+        return out
+
+    def set_params(self, params) -> None:
+        import numpy as np
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.n_params()
+        check(lib().mfx_nn_set_params(self._h, p.ctypes.data))
+
+    @property
+    def output_scale(self) -> float:
+        return float(lib().mfx_nn_output_scale(self._h))
+
+    @output_scale.setter
+    def output_scale(self, s: float) -> None:
+        check(lib().mfx_nn_set_output_scale(self._h, s))
+
+    def set_mode(self, smoother_degree: int, use_network: bool = True) -> None:
+        """0: the network alone; k > 0: k Chebyshev(Jacobi) steps, the network
+        on the remaining residual, k steps again (needs an operator built
+        with with_fp32); use_network=False keeps only the Chebyshev steps."""
+        check(lib().mfx_nn_set_mode(self._h, smoother_degree, int(use_network)))
+
+    @staticmethod
+    def _dims(dims):
+        return (C.c_uint32 * 3)(*[int(d) for d in dims])
+
+    def forward(self, dims, x: torch.Tensor) -> torch.Tensor:
+        """net(x) on a grid of dims = (nx, ny, nz) nodes (FP32 fields)."""
+        y = torch.empty_like(x)
+        check(lib().mfx_nn_forward(self._h, self._dims(dims), _ptr(x), _ptr(y), _stream()))
+        return y
+
+    def gradient(self, dims, x: torch.Tensor, dy: torch.Tensor) -> torch.Tensor:
+        g = torch.empty(self.n_params(), dtype=torch.float32, device="cuda")
+        check(lib().mfx_nn_gradient(self._h, self._dims(dims), _ptr(x), _ptr(dy), _ptr(g),
+                                    _stream()))
+        return g
+
+    def train_step(self, dims, x: torch.Tensor, target: torch.Tensor, lr: float) -> float:
+        """One Adam step on the mean squared error over a batch of grids
+        (x, target: [batch, nz, ny, nx] FP32); returns the loss before it."""
+        batch = x.shape[0]
+        loss = C.c_double()
+        check(lib().mfx_nn_train_step(self._h, self._dims(dims), batch, _ptr(x), _ptr(target),
+                                      lr, C.byref(loss), _stream()))
+        return loss.value
+
+    def train_energy_step(self, op: LaplaceOperator, r: torch.Tensor, e: torch.Tensor,
+                          lr: float) -> float:
+        """One Adam step on the energy-norm loss of whole fields (r, e:
+        [batch, N] FP64, A e = r); returns the loss before it."""
+        loss = C.c_double()
+        check(lib().mfx_nn_train_energy(self._h, op._h, r.shape[0], _ptr(r), _ptr(e), lr,
+                                        C.byref(loss), _stream()))
+        return loss.value
+
+    def vmult(self, op: LaplaceOperator, z: torch.Tensor, r: torch.Tensor) -> None:
+        check(lib().mfx_nn_vmult(self._h, op._h, _ptr(z), _ptr(r), _stream()))
 
 
 def batch_instances(count: int = 64, seed: int = 2, n_min: int = 4, n_max: int = 32,

@@ -1,0 +1,124 @@
+"""Training data and training loop of the learned preconditioner
+(benchmarks_amd.laplace.LearnedPreconditioner).
+
+Data: residual -> error pairs (r, e = A^-1 r) on the node grid of a box
+operator.  r is the nodal interpolant of a sum of `n_modes` random Fourier
+modes sin(pi k1 x) sin(pi k2 y) sin(pi k3 z), integer wave numbers in 1..8
+and amplitudes uniform in [-1, 1] drawn from the SplitMix64 stream of
+`seed`, zero on the Dirichlet boundary; e solves A e = r in FP64 (multigrid
+CG when a PreconditionMG is given, else Jacobi-CG) to 1e-10.
+
+Training: the network sees x = r_c / ||r_c|| on crops r_c of the fields and
+learns t = e_c / (s ||r_c||), s = mean ||e|| / ||r|| over the fields (the
+output scale); mean squared error, Adam, all through the library's kernels.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import torch
+
+from .laplace import (DiagonalMatrix, PreconditionMG, ReductionControl, SolverCG)
+
+
+def _stream(seed: int):
+    """u in [0, 1) from SplitMix64: z = seed + k 0x9E3779B97F4A7C15, standard mix."""
+    M = (1 << 64) - 1
+    k = 0
+    while True:
+        k += 1
+        z = (seed + k * 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        yield (z >> 11) * 2.0 ** -53
+
+
+def smooth_rhs(op, xyz, u, n_modes):
+    f = torch.zeros(op.m(), dtype=torch.float64, device="cuda")
+    for _ in range(n_modes):
+        k = [1 + min(int(next(u) * 8), 7) for _ in range(3)]
+        a = 2.0 * next(u) - 1.0
+        f += a * (torch.sin(math.pi * k[0] * xyz[:, 0]) * torch.sin(math.pi * k[1] * xyz[:, 1])
+                  * torch.sin(math.pi * k[2] * xyz[:, 2]))
+    op.zero_constrained(f)
+    return f
+
+
+def smooth_fields(op_or_mg, count: int, seed: int = 3, n_modes: int = 32, rel_tol: float = 1e-10):
+    """`count` pairs (r, e) as FP64 vectors of the operator's DoFs."""
+    if isinstance(op_or_mg, PreconditionMG):
+        op, precond = op_or_mg.operator(), op_or_mg
+    else:
+        op, precond = op_or_mg, DiagonalMatrix(op_or_mg.compute_inverse_diagonal())
+    xyz = op.dof_coordinates()
+    u = _stream(seed)
+    rs, es = [], []
+    for _ in range(count):
+        r = smooth_rhs(op, xyz, u, n_modes)
+        e = op.initialize_dof_vector()
+        SolverCG(ReductionControl(100000, 1e-300, rel_tol)).solve(op, e, r, precond)
+        rs.append(r)
+        es.append(e)
+    return rs, es
+
+
+def train_on_fields(net, dims, rs, es, steps: int, batch: int, crop, lr: float, seed: int = 1):
+    """Adam steps on random crops (edge `crop` nodes, or whole fields when
+    crop is None) of the (r, e) fields; dims = (nx, ny, nz) of the fields.
+    Sets the network's output scale.  Returns loss statistics and timing."""
+    nx, ny, nz = dims
+    ratios = [(torch.linalg.norm(e) / torch.linalg.norm(r)).item() for r, e in zip(rs, es)]
+    s = sum(ratios) / len(ratios)
+    net.output_scale = s
+    R = [r.view(nz, ny, nx) for r in rs]
+    E = [e.view(nz, ny, nx) for e in es]
+    c = (nx, ny, nz) if crop is None else (min(crop, nx), min(crop, ny), min(crop, nz))
+    u = _stream(seed + 1000)
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        xs, ts = [], []
+        for _ in range(batch):
+            f = min(int(next(u) * len(R)), len(R) - 1)
+            ox = min(int(next(u) * (nx - c[0] + 1)), nx - c[0])
+            oy = min(int(next(u) * (ny - c[1] + 1)), ny - c[1])
+            oz = min(int(next(u) * (nz - c[2] + 1)), nz - c[2])
+            rc = R[f][oz:oz + c[2], oy:oy + c[1], ox:ox + c[0]]
+            ec = E[f][oz:oz + c[2], oy:oy + c[1], ox:ox + c[0]]
+            nr = torch.linalg.norm(rc).clamp_min(1e-300)
+            xs.append((rc / nr).float())
+            ts.append((ec / (s * nr)).float())
+        x = torch.stack(xs).contiguous()
+        t = torch.stack(ts).contiguous()
+        losses.append(net.train_step(c, x, t, lr))
+    torch.cuda.synchronize()
+    k = max(1, min(10, len(losses) // 4))
+    return {"steps": steps, "batch": batch, "crop": c, "seconds": time.perf_counter() - t0,
+            "loss_first": sum(losses[:k]) / k, "loss_last": sum(losses[-k:]) / k,
+            "output_scale": s}
+
+
+def train_energy(net, op, rs, es, steps: int, batch: int, lr: float, seed: int = 1):
+    """Adam steps on the energy-norm loss (pred - e)^T A (pred - e) / e^T A e
+    of whole fields of `op` (the loss the preconditioner is judged by);
+    keeps the output scale already set (or sets it from the fields)."""
+    if net.output_scale == 1.0:
+        ratios = [(torch.linalg.norm(e) / torch.linalg.norm(r)).item() for r, e in zip(rs, es)]
+        net.output_scale = sum(ratios) / len(ratios)
+    u = _stream(seed + 2000)
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        idx = [min(int(next(u) * len(rs)), len(rs) - 1) for _ in range(batch)]
+        r = torch.stack([rs[i] for i in idx]).contiguous()
+        e = torch.stack([es[i] for i in idx]).contiguous()
+        losses.append(net.train_energy_step(op, r, e, lr))
+    torch.cuda.synchronize()
+    k = max(1, min(10, len(losses) // 4))
+    return {"steps": steps, "batch": batch, "seconds": time.perf_counter() - t0,
+            "loss_first": sum(losses[:k]) / k, "loss_last": sum(losses[-k:]) / k,
+            "output_scale": net.output_scale}

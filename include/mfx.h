@@ -357,6 +357,60 @@ mfx_status mfx_batch_cg_solve(mfx_batch *batch, double *x, const double *b,
                               double rel_tol, int use_graph, mfx_cg_result *results,
                               void *stream);
 
+/* ----------------------------------------------------------------------
+ * Learned preconditioner.  No counterpart in the reference.  A 4-level 3D
+ * U-Net (1 -> 16 -> 32 -> 64 channels, 3^3 kernels, additive skips, ReLU,
+ * no biases; 360288 FP32 master weights in layer order, each [cout][27][cin])
+ * on the node grid of a box operator, mapping a residual r to an error
+ * estimate: z = ||r|| s net(r / ||r||).  Forward on BF16 MFMA with FP32
+ * accumulation, backward in FP32, training with Adam (b1 0.9, b2 0.999),
+ * all on the GPU.  Grids are node counts {nx, ny, nz}; fields are FP32
+ * device arrays, x fastest.
+ * ---------------------------------------------------------------------- */
+typedef struct mfx_nn mfx_nn;
+
+uint64_t mfx_nn_n_params(void);
+/* He-normal weights from the SplitMix64 stream of seed */
+mfx_status mfx_nn_create(uint64_t seed, mfx_nn **out);
+mfx_status mfx_nn_destroy(mfx_nn *nn);
+mfx_status mfx_nn_get_params(const mfx_nn *nn, float *host_params);
+mfx_status mfx_nn_set_params(mfx_nn *nn, const float *host_params);
+mfx_status mfx_nn_set_output_scale(mfx_nn *nn, double scale);
+double mfx_nn_output_scale(const mfx_nn *nn);
+/* Preconditioner mode: smoother_degree 0 applies the network alone; k > 0
+ * runs k Chebyshev(Jacobi) steps (FP32, interval [lambda_max/15, lambda_max],
+ * lambda_max = 1.2 x a 12-step Lanczos estimate; the operator needs
+ * with_fp32), then the network on the remaining residual as a coarse
+ * correction, then k Chebyshev steps again.  use_network = 0 keeps only the
+ * Chebyshev steps (the method without the network). */
+mfx_status mfx_nn_set_mode(mfx_nn *nn, int smoother_degree, int use_network);
+/* y = net(x) on one grid */
+mfx_status mfx_nn_forward(mfx_nn *nn, const uint32_t dims[3], const float *x, float *y,
+                          void *stream);
+/* grad (device, n_params floats) = d(sum dy . net(x)) / d(weights) */
+mfx_status mfx_nn_gradient(mfx_nn *nn, const uint32_t dims[3], const float *x, const float *dy,
+                           float *grad, void *stream);
+/* one Adam step on the mean over `batch` grids of the mean squared error of
+ * net(x_b) against target_b (x, target: batch consecutive grids); *loss
+ * receives that mean before the step */
+mfx_status mfx_nn_train_step(mfx_nn *nn, const uint32_t dims[3], int batch, const float *x,
+                             const float *target, double lr, double *loss, void *stream);
+/* one Adam step on the energy-norm loss over `batch` whole fields of a box
+ * operator: mean of (pred - e)^T A (pred - e) / e^T A e with
+ * pred = ||r|| s net(r / ||r||); r, e: batch consecutive FP64 DoF vectors
+ * with A e = r; *loss receives that mean before the step */
+mfx_status mfx_nn_train_energy(mfx_nn *nn, mfx_op *op, int batch, const double *r,
+                               const double *e, double lr, double *loss, void *stream);
+/* preconditioner application on a box operator's node grid (FP64 in/out,
+ * constrained rows pass through) */
+mfx_status mfx_nn_vmult(mfx_nn *nn, mfx_op *op, double *z, const double *r, void *stream);
+/* flexible CG (up to 4 stored A-orthogonal directions) on op preconditioned
+ * by the network;
+ * stopping rule and result fields of mfx_cg_solve */
+mfx_status mfx_nn_fcg_solve(mfx_nn *nn, mfx_op *op, double *x, const double *b, int max_steps,
+                            double abs_tol, double rel_tol, mfx_cg_result *result,
+                            void *stream);
+
 #ifdef __cplusplus
 }
 #endif

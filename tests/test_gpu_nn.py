@@ -1,0 +1,176 @@
+"""The learned preconditioner's network on the GPU (mfx_nn_*) against the
+FP64 network oracle (oracle/nn_oracle.py).
+
+Tolerances: the forward pass stores activations and weights in bfloat16
+(8-bit mantissa); against the oracle with the same bfloat16 storage it
+agrees to 1e-2 relative (2-norm) -- rare one-ulp rounding differences of an
+activation propagate -- and to 4e-2 against plain FP64.  Weight gradients
+(FP32 backward on the stored activations) agree per layer to 2e-2 with the
+bfloat16-storage oracle.  Training and flexible CG are checked by what they
+must achieve: a falling loss, and fewer FCG steps with the trained network
+than without it."""
+import numpy as np
+import pytest
+
+from oracle import nn_oracle as N
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (13, 11, 9)  # x, y, z nodes
+
+
+def _t(a):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device="cuda")
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _net(seed=3):
+    from benchmarks_amd.laplace import LearnedPreconditioner
+    net = LearnedPreconditioner(seed)
+    flat = N.init_params(seed)
+    net.set_params(flat)
+    return net, flat.astype(np.float32).astype(np.float64)
+
+
+def test_initialization_matches_oracle(gpu):
+    from benchmarks_amd.laplace import LearnedPreconditioner
+    net = LearnedPreconditioner(5)
+    assert net.n_params() == N.n_params()
+    assert np.allclose(net.get_params(), N.init_params(5), rtol=1e-6, atol=1e-7)
+
+
+def test_forward_matches_oracle(gpu):
+    net, flat = _net()
+    x = np.random.default_rng(1).standard_normal(DIMS[::-1])
+    y = net.forward(DIMS, _t(x)).cpu().numpy().astype(np.float64)
+    yb, _ = N.forward(N.split_params(flat), x, bf16_storage=True)
+    y64, _ = N.forward(N.split_params(flat), x)
+    assert _rel(y, yb) <= 1e-2
+    assert _rel(y, y64) <= 4e-2
+
+
+def test_forward_on_odd_and_even_grids(gpu):
+    net, flat = _net(4)
+    for dims in [(8, 8, 8), (17, 5, 6), (2, 3, 4)]:
+        x = np.random.default_rng(2).standard_normal(dims[::-1])
+        y = net.forward(dims, _t(x)).cpu().numpy().astype(np.float64)
+        yb, _ = N.forward(N.split_params(flat), x, bf16_storage=True)
+        assert _rel(y, yb) <= 1e-2, dims
+
+
+def test_gradient_matches_oracle(gpu):
+    net, flat = _net()
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(DIMS[::-1])
+    w = rng.standard_normal(DIMS[::-1])
+    g = net.gradient(DIMS, _t(x), _t(w)).cpu().numpy().astype(np.float64)
+    P = {k: N.bf16(v) for k, v in N.split_params(flat).items()}
+    _, cache = N.forward(P, x, bf16_storage=True)
+    go = N.flatten(N.backward(P, cache, w))
+    starts = np.cumsum([0] + [27 * ci * co for _, ci, co in N.LAYERS])
+    for i, (name, _, _) in enumerate(N.LAYERS):
+        a, b = starts[i], starts[i + 1]
+        assert _rel(g[a:b], go[a:b]) <= 2e-2, name
+
+
+def test_network_is_positively_homogeneous_on_gpu(gpu):
+    net, _ = _net()
+    x = _t(np.random.default_rng(4).standard_normal(DIMS[::-1]))
+    y1 = net.forward(DIMS, x)
+    y2 = net.forward(DIMS, 4.0 * x)  # a power of two: the bf16 roundings scale exactly
+    assert (y2 - 4.0 * y1).abs().max().item() <= 1e-6 * y1.abs().max().item()
+
+
+def test_training_reduces_loss(gpu):
+    import torch
+    net, _ = _net(7)
+    rng = np.random.default_rng(5)
+    dims = (12, 12, 12)
+    x = _t(rng.standard_normal((4,) + dims))
+    # a smooth linear target map: a 3-point average along x of the input
+    xa = x.cpu().numpy()
+    t = _t((np.roll(xa, 1, axis=3) + xa + np.roll(xa, -1, axis=3)) / 3.0)
+    losses = [net.train_step(dims, x, t, 2e-3) for _ in range(150)]
+    assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
+    assert torch.isfinite(torch.tensor(losses)).all()
+
+
+def _problem(p=2, n=10):
+    from benchmarks_amd.laplace import LaplaceOperator
+    return LaplaceOperator(p, p + 1, O.GLL, n, deform=0.05)
+
+
+def test_preconditioner_passes_constrained_rows(gpu):
+    from benchmarks_amd.laplace import LearnedPreconditioner, random_vector
+    A = _problem()
+    net = LearnedPreconditioner(2)
+    r = random_vector(A.m(), 3)
+    z = A.initialize_dof_vector()
+    net.vmult(A, z, r)
+    cons = A.device_array("constrained", __import__("torch").int32, A.info.n_constrained).long()
+    assert (z[cons] == r[cons]).all()
+    z2 = A.initialize_dof_vector()
+    net.vmult(A, z2, 3.0 * r)  # z = ||r|| s net(r / ||r||): exactly linear in ||r||
+    assert ((z2 - 3.0 * z).abs().max() / z.abs().max()).item() <= 1e-12
+
+
+def test_trained_network_reduces_fcg_steps(gpu):
+    """Train on (residual, error) pairs of the operator itself with the
+    energy-norm loss, then compare flexible-CG steps of the two-level
+    preconditioner (one Chebyshev step on each side of the network) with the
+    same cycle without the network and with Jacobi: what the trained network
+    adds.  Measured: 22 steps against 66 without the network, 118 Jacobi."""
+    from benchmarks_amd.laplace import (DiagonalMatrix, LaplaceOperator, LearnedPreconditioner,
+                                        NoConvergence, ReductionControl, SolverCG, random_vector)
+    from benchmarks_amd.learned import smooth_fields, train_energy
+    A = LaplaceOperator(2, 3, O.GLL, 12, deform=0.05, with_fp32=True)
+    inv = A.compute_inverse_diagonal()
+    rs, es = smooth_fields(A, 24, seed=3, n_modes=32)
+    trained = LearnedPreconditioner(11)
+    stats = train_energy(trained, A, rs, es, steps=400, batch=4, lr=2e-3, seed=1)
+    assert stats["loss_last"] < 0.1 * stats["loss_first"], stats
+    without = LearnedPreconditioner(11)
+    trained.set_mode(1, True)
+    without.set_mode(1, False)
+    b = random_vector(A.m(), 21)
+    A.zero_constrained(b)
+    steps = {}
+    for name, P in [("trained", trained), ("without", without), ("jacobi", DiagonalMatrix(inv))]:
+        x = A.initialize_dof_vector()
+        c = ReductionControl(3000, 1e-300, 1e-8)
+        SolverCG(c).solve(A, x, b, P)
+        assert c.last_value() <= 1e-8 * c.initial_value()
+        steps[name] = c.last_step()
+    print("FCG steps", steps)
+    assert steps["trained"] < 0.6 * steps["without"], steps
+    assert steps["trained"] < 0.5 * steps["jacobi"], steps
+
+
+def test_flexible_cg_equals_pcg_for_linear_preconditioner(gpu):
+    """With the network switched off the preconditioner is a fixed Chebyshev
+    polynomial, so flexible CG must take PCG's steps: compare with the
+    oracle's Jacobi-CG bound (it cannot be slower than 1 Chebyshev step per
+    Jacobi step) and with a rerun (deterministic)."""
+    from benchmarks_amd.laplace import (DiagonalMatrix, LaplaceOperator, LearnedPreconditioner,
+                                        ReductionControl, SolverCG, random_vector)
+    A = LaplaceOperator(3, 4, O.GLL, 6, deform=0.05, with_fp32=True)
+    b = random_vector(A.m(), 2)
+    A.zero_constrained(b)
+    P = LearnedPreconditioner(1)
+    P.set_mode(2, False)
+    runs = []
+    for _ in range(2):
+        x = A.initialize_dof_vector()
+        c = ReductionControl(3000, 1e-300, 1e-9)
+        SolverCG(c).solve(A, x, b, P)
+        runs.append(c.last_step())
+    xj = A.initialize_dof_vector()
+    cj = ReductionControl(3000, 1e-300, 1e-9)
+    SolverCG(cj).solve(A, xj, b, DiagonalMatrix(A.compute_inverse_diagonal()))
+    assert runs[0] == runs[1]
+    assert runs[0] < cj.last_step()
