@@ -19,7 +19,10 @@ the mixed-precision BP3 solve (Q4, Gauss q=6, deformed 128^3 cells, 135M
 DoFs, manufactured-solution right-hand side, FP64 CG to 1e-10 with an FP32
 p/h-multigrid V-cycle) with its error against an FP64 solution, and the
 batched solve of 64 BP5 problems of uneven size and degree (per-instance
-Jacobi-CG in one hipGraph-replayed batch) against solving them one by one.
+Jacobi-CG in one hipGraph-replayed batch) against solving them one by one,
+and the learned-preconditioner flexible CG (Q4 BP5 on 64^3 cells) with its
+training cost, against the same method without the network, Jacobi-CG and
+multigrid CG.
 """
 import argparse
 import ctypes as C
@@ -315,6 +318,62 @@ def bench_batch(count=64, seed=2):
     return out
 
 
+def bench_learned(n=64, p=4, n_fields=32, mse_steps=200, energy_steps=600, degree=4, seed=3):
+    """Learned-preconditioner FCG (configuration C5): Q4 BP5 (GLL, on-the-fly
+    geometry) on n^3 deformed cells.  Two-grid preconditioner: `degree`
+    Chebyshev(Jacobi) steps (FP32), the U-Net as the coarse solve on the
+    node grid of Q1 on (n/2)^3 cells (restricted residual in, prolongated
+    correction out, energy-optimal weight), `degree` Chebyshev steps again.
+    The network is trained on (residual, error) pairs of that Q1 operator
+    (Jacobi-CG solves of `n_fields` smooth right-hand sides, 32 random
+    Fourier modes, wave numbers up to n/4, SplitMix64 seed 3): mean squared
+    error, then the energy-norm loss.  Flexible CG to 1e-8 on a fresh smooth
+    right-hand side, next to the same cycle without the network, Jacobi-CG
+    and multigrid CG; errors against an FP64 solution certified to 1e-12."""
+    import torch
+
+    from benchmarks_amd.laplace import (DiagonalMatrix, LaplaceOperator, LearnedPreconditioner,
+                                        PreconditionMG)
+    from benchmarks_amd.learned import _stream, smooth_rhs, train_coarse
+    mg = PreconditionMG(p, p + 1, 1, n, deform=0.05, geometry=ON_THE_FLY)
+    A = mg.operator()
+    coarse = LaplaceOperator(1, 2, 1, n // 2, deform=0.05, with_fp32=True)
+    net = LearnedPreconditioner(seed)
+    st = train_coarse(net, coarse, n_fields=n_fields, mse_steps=mse_steps,
+                      energy_steps=energy_steps, seed=seed)
+    b = smooth_rhs(A, A.dof_coordinates(), _stream(seed + 1), 32)
+    x_ref = A.initialize_dof_vector()
+    _timed_solve(A, x_ref, b, mg, 1e-12, 200)
+    net.set_mode(degree, True)
+    without = LearnedPreconditioner(seed)
+    without.set_mode(degree, False)
+    x = A.initialize_dof_vector()
+    out = {"workload": f"BP5 Q{p} GLL, deformed {n}^3 cells, smooth RHS (32 Fourier modes), "
+                       f"flexible CG to 1e-8",
+           "n_dofs": A.m(),
+           "network_grid": f"Q1 on {n // 2}^3 cells ({st['coarse_dims'][0]}^3 nodes)",
+           "training": f"{n_fields} Jacobi-CG solves on the coarse grid (k_max {st['k_max']}), "
+                       f"{mse_steps} MSE steps (batch 8) + {energy_steps} energy-norm steps "
+                       f"(batch 2)",
+           "data_s": st["data_s"], "train_mse_s": st["mse"]["seconds"],
+           "train_energy_s": st["energy"]["seconds"],
+           "mse_loss_first_last": [st["mse"]["loss_first"], st["mse"]["loss_last"]],
+           "energy_loss_first_last": [st["energy"]["loss_first"], st["energy"]["loss_last"]],
+           "smoother_degree": degree}
+    for name, P, steps in [("learned_fcg", net, 2000), ("chebyshev_fcg_without_network", without, 5000),
+                           ("jacobi_cg", DiagonalMatrix(A.compute_inverse_diagonal()), 100000),
+                           ("multigrid_cg", mg, 200)]:
+        _timed_solve(A, x, b, P, 1e-4, steps)  # warm-up
+        t, c = _timed_solve(A, x, b, P, 1e-8, steps)
+        out[name] = {"steps": c.last_step(), "seconds": t, "err_vs_fp64": _rel(x, x_ref)}
+    out["learned_speedup_vs_jacobi"] = out["jacobi_cg"]["seconds"] / out["learned_fcg"]["seconds"]
+    out["learned_speedup_vs_chebyshev"] = (out["chebyshev_fcg_without_network"]["seconds"]
+                                          / out["learned_fcg"]["seconds"])
+    del mg, A, net, without, x, x_ref, b, coarse
+    torch.cuda.empty_cache()
+    return out
+
+
 def stream_copy_GBs(n=1 << 28, reps=10):
     """Measured HBM ceiling: device-to-device copy of 2 GiB (read + write bytes)."""
     import torch
@@ -409,6 +468,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 135M-DoF mixed-precision solve")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched solve")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the multi-GPU path (slab decomposition with halo "
@@ -472,6 +532,8 @@ def main():
         out["mixed_precision_bp3"] = bench_c3()
     if not args.no_cg and not args.no_batch:
         out["batched_solves"] = bench_batch()
+    if not args.no_cg and not args.no_learned:
+        out["learned_preconditioner"] = bench_learned()
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))

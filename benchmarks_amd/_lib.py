@@ -140,6 +140,9 @@ _SIGS = {
     "mfx_nn_set_output_scale": (C.c_int, [_P, C.c_double]),
     "mfx_nn_output_scale": (C.c_double, [_P]),
     "mfx_nn_set_mode": (C.c_int, [_P, C.c_int, C.c_int]),
+    "mfx_nn_set_coarse": (C.c_int, [_P, _P]),
+    "mfx_nn_smoother_info": (C.c_int, [_P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), _P]),
     "mfx_nn_forward": (C.c_int, [_P, C.POINTER(C.c_uint32), _P, _P, _P]),
     "mfx_nn_gradient": (C.c_int, [_P, C.POINTER(C.c_uint32), _P, _P, _P, _P]),
     "mfx_nn_train_step": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_int, _P, _P, C.c_double,

@@ -29,6 +29,9 @@
 //     energy-optimal step along c: the correction never increases the
 //     energy norm of the error), then k Chebyshev steps from z.  Without the
 //     network the same cycle is a plain Chebyshev polynomial preconditioner.
+//     With a coarse operator set (mfx_nn_set_coarse) the network runs on its
+//     node grid: rho is restricted (R = P^T of the nodal interpolation
+//     between the two box operators), the network's output prolongated.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -38,6 +41,7 @@
 #include "mg.hpp"
 #include "nn.hpp"
 #include "op.hpp"
+#include "transfer.hpp"
 
 namespace mfx {
 namespace {
@@ -99,6 +103,12 @@ struct mfx_nn {
   DBuf<float> dinv32, r32, z32, t32, c32, n32, an32;
   DBuf<double> tmp64, tmp64b, tmp64c;
   double last_omega = 0.0;
+  // coarse correction: the network on another box operator's node grid
+  mfx_op *coarse_op = nullptr;
+  const mfx_op *tr_fine = nullptr;
+  Transfer tr;
+  DBuf<float> rc32, ec32, ta32, tb32;
+  DBuf<double> rc64;
 
   void setup_smoother(mfx_op &op, hipStream_t s) {
     if (sm_op == &op && sm_n == op.N) return;
@@ -249,13 +259,56 @@ struct mfx_nn {
     launch_copy_listed<double>(z, r, op.cons.ptr, op.n_cons, s);
   }
 
+  // z32 += omega P c with c = ||rho_c|| s net(rho_c / ||rho_c||) on the
+  // coarse node grid, rho_c = R rho (rho in t32), omega energy-optimal
+  void coarse_correction(mfx_op &op, hipStream_t s) {
+    mfx_op &C = *coarse_op;
+    const uint64_t N = op.N, Nc = C.N;
+    if (tr_fine != &op) {
+      build_transfer(op, C, tr, true, false);
+      rc32.alloc(Nc);
+      ec32.alloc(Nc);
+      rc64.alloc(Nc);
+      ta32.alloc(N);
+      tb32.alloc(N);
+      tr_fine = &op;
+    }
+    restrict_<float>(tr.R32, tr.fine, tr.coarse, t32.ptr, ta32.ptr, tb32.ptr, rc32.ptr, s);
+    launch_set_listed_f32(rc32.ptr, 0.0f, C.cons.ptr, C.n_cons, s);
+    launch_to_double(rc32.ptr, rc64.ptr, Nc, s);
+    const double rn = std::sqrt(dot64(rc64.ptr, rc64.ptr, Nc, dpart, s));
+    if (!(rn > 0.0)) return;
+    uint32_t d[3];
+    for (int k = 0; k < 3; ++k) d[k] = C.box.n[k] * C.p + 1;
+    ensure(d, false);
+    nn_scale_to_f32(rc64.ptr, x32.ptr, 1.0 / rn, Nc, s);
+    forward(s);
+    launch_fill_f32(ec32.ptr, 0.0f, Nc, s);
+    nn_axpy_f32(ec32.ptr, (float)(rn * out_scale), y32.ptr, Nc, s);
+    launch_set_listed_f32(ec32.ptr, 0.0f, C.cons.ptr, C.n_cons, s);
+    prolongate<float>(tr.P32, tr.fine, tr.coarse, ec32.ptr, ta32.ptr, tb32.ptr, n32.ptr, false, s);
+    launch_set_listed_f32(n32.ptr, 0.0f, op.cons.ptr, op.n_cons, s);
+    vmult32(op, an32.ptr, n32.ptr, s);
+    launch_to_double(t32.ptr, tmp64.ptr, N, s);
+    launch_to_double(n32.ptr, tmp64b.ptr, N, s);
+    launch_to_double(an32.ptr, tmp64c.ptr, N, s);
+    const double crho = dot64(tmp64b.ptr, tmp64.ptr, N, dpart, s);
+    const double cac = dot64(tmp64b.ptr, tmp64c.ptr, N, dpart, s);
+    last_omega = cac > 0.0 ? crho / cac : 0.0;
+    if (last_omega != 0.0) nn_axpy_f32(z32.ptr, (float)last_omega, n32.ptr, N, s);
+  }
+
   void precondition_two_level(mfx_op &op, double *z, const double *r, hipStream_t s) {
     setup_smoother(op, s);
     const uint64_t N = op.N;
     launch_to_float(r, r32.ptr, N, s);
     launch_set_listed_f32(r32.ptr, 0.0f, op.cons.ptr, op.n_cons, s);
     chebyshev(op, z32.ptr, r32.ptr, dinv32.ptr, t32.ptr, c32.ptr, smooth_degree, lmin, lmax, true, s);
-    if (use_net) {
+    if (use_net && coarse_op) {
+      vmult32(op, t32.ptr, z32.ptr, s);
+      launch_residual_f32(t32.ptr, r32.ptr, t32.ptr, N, s);
+      coarse_correction(op, s);
+    } else if (use_net) {
       uint32_t d[3];
       for (int k = 0; k < 3; ++k) d[k] = op.box.n[k] * op.p + 1;
       ensure(d, false);
@@ -376,6 +429,27 @@ mfx_status mfx_nn_set_output_scale(mfx_nn *nn, double scale) {
   });
 }
 double mfx_nn_output_scale(const mfx_nn *nn) { return nn ? nn->out_scale : 0.0; }
+
+mfx_status mfx_nn_set_coarse(mfx_nn *nn, mfx_op *coarse) {
+  return guard([&] {
+    MFX_REQUIRE(nn, "null pointer argument");
+    MFX_REQUIRE(!coarse || coarse->is_box, "the coarse operator must come from mfx_op_create_box");
+    nn->coarse_op = coarse;
+    nn->tr_fine = nullptr;
+  });
+}
+
+mfx_status mfx_nn_smoother_info(mfx_nn *nn, mfx_op *op, double *lambda_min, double *lambda_max,
+                                double *last_omega, void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(nn && op && lambda_min && lambda_max && last_omega, "null pointer argument");
+    MFX_REQUIRE(op->is_box, "the preconditioner needs an operator from mfx_op_create_box");
+    nn->setup_smoother(*op, as_stream(stream));
+    *lambda_min = nn->lmin;
+    *lambda_max = nn->lmax;
+    *last_omega = nn->last_omega;
+  });
+}
 
 mfx_status mfx_nn_set_mode(mfx_nn *nn, int smoother_degree, int use_network) {
   return guard([&] {

@@ -57,6 +57,8 @@ class LaplaceOperator:
         self._h = None
         # a handle owned by another object (a multigrid level) is not destroyed here
         self._owner = _owner
+        # cells per direction of a box operator (None for general meshes)
+        self.box_cells = None
         if _handle is not None:
             self._h = _handle
         else:
@@ -74,6 +76,7 @@ class LaplaceOperator:
             h = C.c_void_p()
             check(lib().mfx_op_create_box(C.byref(d), C.byref(h)))
             self._h = h
+            self.box_cells = tuple(int(c) for c in n_cells)
         info = _lib.OpInfo()
         check(lib().mfx_op_get_info(self._h, C.byref(info)))
         self.info = info
@@ -279,6 +282,7 @@ class PreconditionMG:
         info = _lib.MgLevelInfo()
         check(lib().mfx_mg_get_level(self._h, i, C.byref(info)))
         op = LaplaceOperator(0, 0, _handle=C.c_void_p(info.op), _owner=self)
+        op.box_cells = tuple(int(c) for c in info.n_cells)
         return {"degree": info.degree, "n_q_points_1d": info.n_q_points_1d,
                 "n_cells": tuple(info.n_cells), "n_dofs": int(info.n_dofs),
                 "lambda_max": info.lambda_max, "lambda_min": info.lambda_min, "operator": op}
@@ -497,6 +501,20 @@ class LearnedPreconditioner:
         on the remaining residual, k steps again (needs an operator built
         with with_fp32); use_network=False keeps only the Chebyshev steps."""
         check(lib().mfx_nn_set_mode(self._h, smoother_degree, int(use_network)))
+
+    def set_coarse(self, coarse) -> None:
+        """Run the network on a coarser box operator's node grid (restricted
+        residual in, prolongated correction out); None: the fine grid."""
+        self._coarse = coarse  # keep the operator alive while in use
+        check(lib().mfx_nn_set_coarse(self._h, None if coarse is None else coarse._h))
+
+    def smoother_info(self, op: LaplaceOperator) -> dict:
+        """Chebyshev interval on `op` and the weight of the network's
+        correction in the last application."""
+        lo, hi, om = C.c_double(), C.c_double(), C.c_double()
+        check(lib().mfx_nn_smoother_info(self._h, op._h, C.byref(lo), C.byref(hi), C.byref(om),
+                                         _stream()))
+        return {"lambda_min": lo.value, "lambda_max": hi.value, "omega": om.value}
 
     @staticmethod
     def _dims(dims):

@@ -3,14 +3,21 @@
 
 Data: residual -> error pairs (r, e = A^-1 r) on the node grid of a box
 operator.  r is the nodal interpolant of a sum of `n_modes` random Fourier
-modes sin(pi k1 x) sin(pi k2 y) sin(pi k3 z), integer wave numbers in 1..8
-and amplitudes uniform in [-1, 1] drawn from the SplitMix64 stream of
+modes sin(pi k1 x) sin(pi k2 y) sin(pi k3 z), integer wave numbers in
+1..k_max (default 8) and amplitudes uniform in [-1, 1] drawn from the SplitMix64 stream of
 `seed`, zero on the Dirichlet boundary; e solves A e = r in FP64 (multigrid
 CG when a PreconditionMG is given, else Jacobi-CG) to 1e-10.
 
 Training: the network sees x = r_c / ||r_c|| on crops r_c of the fields and
 learns t = e_c / (s ||r_c||), s = mean ||e|| / ||r|| over the fields (the
-output scale); mean squared error, Adam, all through the library's kernels.
+output scale); mean squared error, Adam, all through the library's kernels;
+then the energy-norm loss on whole fields.
+
+train_coarse does both on the node grid of a coarse box operator (Q1 on half
+the fine cells in the measured configuration) and attaches the network as
+the coarse solve of the two-grid preconditioner: the wave numbers of the
+training fields then reach half the coarse grid's Nyquist number
+(k_max = cells / 2), which covers what a restricted smoothed residual holds.
 """
 from __future__ import annotations
 
@@ -35,10 +42,10 @@ def _stream(seed: int):
         yield (z >> 11) * 2.0 ** -53
 
 
-def smooth_rhs(op, xyz, u, n_modes):
+def smooth_rhs(op, xyz, u, n_modes, k_max: int = 8):
     f = torch.zeros(op.m(), dtype=torch.float64, device="cuda")
     for _ in range(n_modes):
-        k = [1 + min(int(next(u) * 8), 7) for _ in range(3)]
+        k = [1 + min(int(next(u) * k_max), k_max - 1) for _ in range(3)]
         a = 2.0 * next(u) - 1.0
         f += a * (torch.sin(math.pi * k[0] * xyz[:, 0]) * torch.sin(math.pi * k[1] * xyz[:, 1])
                   * torch.sin(math.pi * k[2] * xyz[:, 2]))
@@ -46,7 +53,8 @@ def smooth_rhs(op, xyz, u, n_modes):
     return f
 
 
-def smooth_fields(op_or_mg, count: int, seed: int = 3, n_modes: int = 32, rel_tol: float = 1e-10):
+def smooth_fields(op_or_mg, count: int, seed: int = 3, n_modes: int = 32, rel_tol: float = 1e-10,
+                  k_max: int = 8):
     """`count` pairs (r, e) as FP64 vectors of the operator's DoFs."""
     if isinstance(op_or_mg, PreconditionMG):
         op, precond = op_or_mg.operator(), op_or_mg
@@ -56,7 +64,7 @@ def smooth_fields(op_or_mg, count: int, seed: int = 3, n_modes: int = 32, rel_to
     u = _stream(seed)
     rs, es = [], []
     for _ in range(count):
-        r = smooth_rhs(op, xyz, u, n_modes)
+        r = smooth_rhs(op, xyz, u, n_modes, k_max)
         e = op.initialize_dof_vector()
         SolverCG(ReductionControl(100000, 1e-300, rel_tol)).solve(op, e, r, precond)
         rs.append(r)
@@ -122,3 +130,31 @@ def train_energy(net, op, rs, es, steps: int, batch: int, lr: float, seed: int =
     return {"steps": steps, "batch": batch, "seconds": time.perf_counter() - t0,
             "loss_first": sum(losses[:k]) / k, "loss_last": sum(losses[-k:]) / k,
             "output_scale": net.output_scale}
+
+
+def train_coarse(net, coarse, n_fields: int = 32, k_max=None, mse_steps: int = 200,
+                 energy_steps: int = 600, mse_batch: int = 8, energy_batch: int = 2,
+                 seed: int = 3):
+    """Train `net` on (r, e) pairs of the box operator `coarse` (Jacobi-CG
+    solves of `n_fields` smooth right-hand sides, wave numbers up to k_max,
+    default half the smallest cell count), mean squared error on whole
+    fields and then the energy-norm loss, and attach it as the coarse solve
+    (net.set_coarse).  Returns data and training statistics."""
+    cells = coarse.box_cells
+    if cells is None:
+        raise ValueError("the coarse operator must be a box operator")
+    dims = tuple(c * coarse.degree + 1 for c in cells)
+    if k_max is None:
+        k_max = max(1, min(cells) // 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rs, es = smooth_fields(coarse, n_fields, seed=seed, n_modes=32, k_max=k_max)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    st_mse = train_on_fields(net, dims, rs, es, steps=mse_steps, batch=mse_batch, crop=None,
+                             lr=2e-3, seed=seed)
+    st_en = train_energy(net, coarse, rs, es, steps=energy_steps, batch=energy_batch, lr=1e-3,
+                         seed=seed)
+    net.set_coarse(coarse)
+    return {"coarse_dims": dims, "k_max": k_max, "n_fields": n_fields, "data_s": t_data,
+            "mse": st_mse, "energy": st_en}

@@ -384,6 +384,19 @@ double mfx_nn_output_scale(const mfx_nn *nn);
  * correction, then k Chebyshev steps again.  use_network = 0 keeps only the
  * Chebyshev steps (the method without the network). */
 mfx_status mfx_nn_set_mode(mfx_nn *nn, int smoother_degree, int use_network);
+/* Coarse correction on another box operator of the same box (cells of the
+ * fine operator an integer refinement of its cells, and/or a lower degree,
+ * e.g. Q4 on n^3 cells over Q1 on (n/2)^3): with smoother_degree > 0 the
+ * network then runs on that operator's node grid, on the restricted
+ * residual, and its output is prolongated (nodal interpolation and its
+ * transpose) before the energy-optimal weighting.  The caller keeps
+ * `coarse` alive; NULL returns to the fine grid. */
+mfx_status mfx_nn_set_coarse(mfx_nn *nn, mfx_op *coarse);
+/* The smoother's Chebyshev interval on `op` (set up on first use) and the
+ * energy-optimal weight of the network's correction in the last
+ * application (0 when the network did not contribute) */
+mfx_status mfx_nn_smoother_info(mfx_nn *nn, mfx_op *op, double *lambda_min, double *lambda_max,
+                                double *last_omega, void *stream);
 /* y = net(x) on one grid */
 mfx_status mfx_nn_forward(mfx_nn *nn, const uint32_t dims[3], const float *x, float *y,
                           void *stream);

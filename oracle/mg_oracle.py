@@ -35,16 +35,17 @@ def lagrange_values(p: int, x: float) -> np.ndarray:
 
 
 def prolongation_1d(nc: int, pc: int, nf: int, pf: int) -> np.ndarray:
-    """Dense 1D prolongation (nf pf + 1) x (nc pc + 1)."""
-    hstep = nf == 2 * nc and pf == pc
-    assert hstep or (nf == nc and pf > pc)
+    """Dense 1D prolongation (nf pf + 1) x (nc pc + 1), nf = r nc: coarse
+    cell ef // r, reference coordinate (ef mod r + xi_f) / r."""
+    assert nf % nc == 0 and (nf > nc or pf > pc)
+    r = nf // nc
     supf, _ = O.gll_rule(pf + 1)
     P = np.zeros((nf * pf + 1, nc * pc + 1))
     for j in range(nf * pf + 1):
         ef = min(j // pf, nf - 1)
         l = j - ef * pf
-        ec = ef // 2 if hstep else ef
-        xi = 0.5 * ((ef % 2) + supf[l]) if hstep else supf[l]
+        ec = ef // r
+        xi = ((ef % r) + supf[l]) / r
         P[j, ec * pc: ec * pc + pc + 1] += lagrange_values(pc, xi)
     return P
 

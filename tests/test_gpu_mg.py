@@ -30,6 +30,7 @@ def _rel(a, b):
     ((3, (2, 2, 2)), (1, (2, 2, 2)), O.GAUSS),
     ((1, (4, 6, 2)), (1, (2, 3, 1)), O.GAUSS),
     ((2, (4, 2, 2)), (2, (2, 1, 1)), O.GLL),
+    ((4, (4, 2, 4)), (1, (2, 1, 2)), O.GAUSS),   # degree and cells at once
 ])
 def test_transfers_match_oracle(gpu, fine, coarse, quad):
     from benchmarks_amd.laplace import LaplaceOperator, transfer_prolongate, transfer_restrict
@@ -54,7 +55,7 @@ def test_transfer_rejects_unrelated_operators(gpu):
     from benchmarks_amd._lib import MfxError
     from benchmarks_amd.laplace import LaplaceOperator, transfer_prolongate
     Af = LaplaceOperator(2, 3, O.GLL, (4, 4, 4))
-    Ac = LaplaceOperator(1, 2, O.GLL, (3, 4, 4))  # neither same cells nor half
+    Ac = LaplaceOperator(1, 2, O.GLL, (3, 4, 4))  # 4 cells are no refinement of 3
     with pytest.raises(MfxError):
         transfer_prolongate(Af, Ac, _t(np.zeros(Ac.m())))
     Ad = LaplaceOperator(1, 2, O.GLL, (4, 4, 4), deform=0.05)  # other mesh

@@ -151,6 +151,103 @@ def test_trained_network_reduces_fcg_steps(gpu):
     assert steps["trained"] < 0.5 * steps["jacobi"], steps
 
 
+def test_coarse_correction_matches_composition(gpu):
+    """The two-grid application with the network on a coarse box operator,
+    recomposed from the library's separately tested pieces: Chebyshev steps
+    (mfx_chebyshev_f32), restriction and prolongation (checked against the
+    multigrid oracle), the network (checked against the network oracle) and
+    the energy-optimal weight omega = c.rho / c.Ac.  The pieces run in FP64
+    here and in FP32 inside the preconditioner: agreement to 1e-4 relative,
+    and omega to 1e-4 (an omitted or misplaced transfer, weight or
+    constraint is an O(1) difference)."""
+    import torch
+    from benchmarks_amd.laplace import (LaplaceOperator, LearnedPreconditioner, chebyshev_f32,
+                                        random_vector, transfer_prolongate, transfer_restrict)
+    A = LaplaceOperator(2, 3, O.GLL, (8, 6, 4), deform=0.05, with_fp32=True)
+    Cop = LaplaceOperator(1, 2, O.GLL, (4, 3, 2), deform=0.05, with_fp32=True)
+    cdims = (5, 4, 3)
+    net = LearnedPreconditioner(5)
+    net.set_coarse(Cop)
+    deg = 2
+    net.set_mode(deg, True)
+    r = random_vector(A.m(), 7)
+    A.zero_constrained(r)
+    z = A.initialize_dof_vector()
+    net.vmult(A, z, r)
+    info = net.smoother_info(A)
+    assert info["omega"] != 0.0
+    cons = A.device_array("constrained", torch.int32, A.info.n_constrained).long()
+    dinv = A.compute_inverse_diagonal().float()
+    r32 = r.float()
+    r32[cons] = 0.0
+    zc = torch.zeros_like(r32)
+    chebyshev_f32(A, dinv, zc, r32, deg, info["lambda_min"], info["lambda_max"], True)
+    az = torch.empty_like(zc)
+    A.vmult(az, zc)
+    rho = (r32 - az).double()
+    rc = transfer_restrict(A, Cop, rho)
+    Cop.zero_constrained(rc)
+    rn = torch.linalg.norm(rc)
+    y = net.forward(cdims, (rc / rn).float().contiguous())
+    ec = (rn * net.output_scale) * y.double()
+    Cop.zero_constrained(ec)
+    c = transfer_prolongate(A, Cop, ec)
+    A.zero_constrained(c)
+    ac = A.initialize_dof_vector()
+    A.vmult(ac, c)
+    omega = (c @ rho / (c @ ac)).item()
+    assert abs(omega - info["omega"]) <= 1e-4 * abs(omega), (omega, info["omega"])
+    zc += (omega * c).float()
+    chebyshev_f32(A, dinv, zc, r32, deg, info["lambda_min"], info["lambda_max"], False)
+    want = zc.double()
+    want[cons] = r[cons]
+    err = (torch.linalg.norm(want - z) / torch.linalg.norm(want)).item()
+    assert err <= 1e-4, err
+    # without the network the coarse operator plays no part (the FP32
+    # operator's atomic scatter leaves the last bits order-dependent)
+    plain = LearnedPreconditioner(5)
+    plain.set_mode(deg, False)
+    z0 = A.initialize_dof_vector()
+    plain.vmult(A, z0, r)
+    net.set_mode(deg, False)
+    z1 = A.initialize_dof_vector()
+    net.vmult(A, z1, r)
+    assert (torch.linalg.norm(z0 - z1) / torch.linalg.norm(z0)).item() <= 1e-6
+
+
+def test_trained_coarse_network_beats_jacobi_in_work(gpu):
+    """The measured configuration at small size: Q4 on 16^3 cells, the
+    network as the coarse solve on the Q1 grid of 8^3 cells, trained on
+    (residual, error) pairs of that Q1 operator (train_coarse).  What the
+    trained network adds: flexible-CG steps against the same cycle without
+    it, and fine-operator applications (2 degree + 2 per FCG step, 1 per
+    Jacobi-CG step) against Jacobi-CG."""
+    from benchmarks_amd.laplace import (DiagonalMatrix, LaplaceOperator, LearnedPreconditioner,
+                                        ReductionControl, SolverCG)
+    from benchmarks_amd.learned import _stream, smooth_rhs, train_coarse
+    A = LaplaceOperator(4, 5, O.GLL, 16, deform=0.05, with_fp32=True, geometry=1)
+    Cop = LaplaceOperator(1, 2, O.GLL, 8, deform=0.05, with_fp32=True)
+    net = LearnedPreconditioner(3)
+    st = train_coarse(net, Cop, n_fields=32, mse_steps=100, energy_steps=300, seed=3)
+    assert st["energy"]["loss_last"] < 0.5 * st["energy"]["loss_first"], st["energy"]
+    deg = 4
+    net.set_mode(deg, True)
+    without = LearnedPreconditioner(3)
+    without.set_mode(deg, False)
+    b = smooth_rhs(A, A.dof_coordinates(), _stream(4), 32)
+    steps = {}
+    for name, P in [("trained", net), ("without", without),
+                    ("jacobi", DiagonalMatrix(A.compute_inverse_diagonal()))]:
+        x = A.initialize_dof_vector()
+        c = ReductionControl(5000, 1e-300, 1e-8)
+        SolverCG(c).solve(A, x, b, P)
+        assert c.last_value() <= 1e-8 * c.initial_value()
+        steps[name] = c.last_step()
+    print("FCG steps", steps)
+    assert steps["trained"] < 0.5 * steps["without"], steps
+    assert (2 * deg + 2) * steps["trained"] < steps["jacobi"], steps
+
+
 def test_flexible_cg_equals_pcg_for_linear_preconditioner(gpu):
     """With the network switched off the preconditioner is a fixed Chebyshev
     polynomial, so flexible CG must take PCG's steps: compare with the

@@ -18,7 +18,8 @@ def _line_coords(n, p):
 
 
 @pytest.mark.parametrize("nc,pc,nf,pf", [(3, 2, 3, 4), (4, 1, 4, 2), (2, 4, 2, 8), (5, 3, 5, 6),
-                                          (3, 1, 6, 1), (2, 2, 4, 2), (3, 4, 6, 4)])
+                                          (3, 1, 6, 1), (2, 2, 4, 2), (3, 4, 6, 4),
+                                          (2, 1, 4, 4), (2, 2, 8, 3), (3, 1, 3, 1 + 3)])
 def test_prolongation_reproduces_coarse_polynomials(nc, pc, nf, pf):
     """A global polynomial of degree pc in the parametric coordinate lies in
     both spaces: P must map its coarse nodal values to its fine ones."""
@@ -35,6 +36,13 @@ def test_prolongation_is_nodal_injection_where_nodes_coincide():
         row = P[2 * m]
         assert row[m] == 1.0 and np.count_nonzero(row) == 1
     assert np.allclose(P[1], [0.5, 0.5, 0, 0, 0])
+
+
+def test_prolongation_rejects_unrelated_lines():
+    with pytest.raises(AssertionError):
+        M.prolongation_1d(3, 1, 4, 2)   # 4 cells are no refinement of 3
+    with pytest.raises(AssertionError):
+        M.prolongation_1d(3, 2, 3, 2)   # same line
 
 
 def test_prolongation_3d_is_exact_for_trilinear_functions():

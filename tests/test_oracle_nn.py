@@ -84,3 +84,13 @@ def test_init_statistics():
     assert flat.size == N.n_params() == 360288
     for name, ci, co in N.LAYERS:
         assert np.std(P[name]) == pytest.approx(np.sqrt(2.0 / (27 * ci)), rel=0.15)
+
+
+def test_bf16_rounding_is_nearest_even():
+    """Against torch's own bfloat16 cast, and on exact ties."""
+    import torch
+    v = _rand(4096, 12) * np.exp(_rand(4096, 13) * 4)
+    want = torch.tensor(v, dtype=torch.float32).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(N.bf16(v), want)
+    ties = np.array([1 + 2.0 ** -8, 1 + 3 * 2.0 ** -8, -(1 + 2.0 ** -8), 2.0 ** -8 * (1 + 2.0 ** -8)])
+    assert np.array_equal(N.bf16(ties), [1.0, 1 + 2.0 ** -6, -1.0, 2.0 ** -8])
