@@ -62,6 +62,13 @@ static void common_init(mfx_op *op, int p, int nq, int quad, int geometry) {
   op->colloc = (quad == MFX_QUAD_GLL && nq == p + 1);
   std::vector<double> B((size_t)nq * (p + 1)), D((size_t)nq * nq), xq(nq), wq(nq), sup(p + 1);
   basis_tables(p, nq, quad, B.data(), D.data(), xq.data(), wq.data(), sup.data());
+  if (op->colloc) {
+    // the patch kernel's compile-time derivative table must be this matrix
+    double mx = 0.0;
+    for (double v : D) mx = std::max(mx, std::fabs(v));
+    MFX_REQUIRE(gll_d_table_error(nq, D.data()) <= 1e-13 * mx,
+                "compile-time GLL derivative table disagrees with the basis tables");
+  }
   op->B.upload(B.data(), B.size());
   op->D.upload(D.data(), D.size());
   op->xq.upload(xq.data(), xq.size());
