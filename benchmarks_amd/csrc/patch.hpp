@@ -38,9 +38,6 @@ struct PatchOp {
 };
 
 PatchShape patch_shape(int p);
-// max |D_table - D| of the patch kernel's compile-time GLL derivative table
-// (n points) against a run-time one (row-major n x n)
-double gll_d_table_error(int n, const double *D);
 uint32_t patch_count(const PatchLayout &lay);
 template <typename T>
 PatchGrid patch_grid(int p, const PatchLayout &lay);

@@ -24,11 +24,21 @@ def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def _ptr(t: torch.Tensor):
+F64, F32, I32 = torch.float64, torch.float32, torch.int32
+
+
+def _ptr(t: torch.Tensor, dtype=None, n: int = None):
+    """Device pointer of a contiguous tensor, after checking its element
+    type (the C entry points take typed pointers: a tensor of another type
+    would be read past its end) and, when given, that it holds n elements."""
     if not t.is_cuda:
         raise ValueError("libmfx works on device vectors; got a host tensor")
     if not t.is_contiguous():
         raise ValueError("vectors must be contiguous")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"expected a {dtype} tensor, got {t.dtype}")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"expected {n} elements, got {t.numel()}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -137,9 +147,9 @@ class LaplaceOperator:
         self._check_vec(dst)
         self._check_vec(src)
         if dst.dtype == torch.float64:
-            check(lib().mfx_op_vmult(self._h, _ptr(dst), _ptr(src), _stream()))
+            check(lib().mfx_op_vmult(self._h, _ptr(dst, F64), _ptr(src, F64), _stream()))
         elif dst.dtype == torch.float32:
-            check(lib().mfx_op_vmult_f32(self._h, _ptr(dst), _ptr(src), _stream()))
+            check(lib().mfx_op_vmult_f32(self._h, _ptr(dst, F32), _ptr(src, F32), _stream()))
         else:
             raise TypeError("vmult supports float64 and float32 vectors")
 
@@ -147,7 +157,7 @@ class LaplaceOperator:
         """dst += A_free src on the unconstrained rows (the element kernel alone)."""
         self._check_vec(dst)
         self._check_vec(src)
-        check(lib().mfx_op_vmult_add(self._h, _ptr(dst), _ptr(src), _stream()))
+        check(lib().mfx_op_vmult_add(self._h, _ptr(dst, F64), _ptr(src, F64), _stream()))
 
     def Tvmult(self, dst, src) -> None:
         self.vmult(dst, src)
@@ -159,25 +169,25 @@ class LaplaceOperator:
     # --- set-up helpers -------------------------------------------------
     def compute_diagonal(self) -> torch.Tensor:
         d = self.initialize_dof_vector()
-        check(lib().mfx_op_diagonal(self._h, _ptr(d), _stream()))
+        check(lib().mfx_op_diagonal(self._h, _ptr(d, F64, self.m()), _stream()))
         return d
 
     def compute_inverse_diagonal(self) -> torch.Tensor:
         d = self.initialize_dof_vector()
-        check(lib().mfx_op_inverse_diagonal(self._h, _ptr(d), _stream()))
+        check(lib().mfx_op_inverse_diagonal(self._h, _ptr(d, F64, self.m()), _stream()))
         return d
 
     def rhs(self, f_kind: int = 0) -> torch.Tensor:
         b = self.initialize_dof_vector()
-        check(lib().mfx_op_rhs(self._h, f_kind, _ptr(b), _stream()))
+        check(lib().mfx_op_rhs(self._h, f_kind, _ptr(b, F64, self.m()), _stream()))
         return b
 
     def zero_constrained(self, v: torch.Tensor) -> None:
-        check(lib().mfx_vec_zero_constrained(self._h, _ptr(v), _stream()))
+        check(lib().mfx_vec_zero_constrained(self._h, _ptr(v, F64, self.m()), _stream()))
 
     def dof_coordinates(self) -> torch.Tensor:
         c = torch.zeros(self.m() * 3, dtype=torch.float64, device="cuda")
-        check(lib().mfx_op_dof_coordinates(self._h, _ptr(c), _stream()))
+        check(lib().mfx_op_dof_coordinates(self._h, _ptr(c, F64, 3 * self.m()), _stream()))
         return c.view(-1, 3)
 
     def device_array(self, name: str, dtype, count: int) -> torch.Tensor:
@@ -295,25 +305,29 @@ class PreconditionMG:
 
     def vmult(self, dst: torch.Tensor, src: torch.Tensor) -> None:
         """dst = one V-cycle applied to src (FP64 in and out)."""
-        check(lib().mfx_mg_vmult(self._h, _ptr(dst), _ptr(src), _stream()))
+        n = self.operator().m()
+        check(lib().mfx_mg_vmult(self._h, _ptr(dst, F64, n), _ptr(src, F64, n), _stream()))
 
 
 def transfer_prolongate(fine: LaplaceOperator, coarse: LaplaceOperator, src: torch.Tensor):
     dst = fine.initialize_dof_vector()
-    check(lib().mfx_transfer_prolongate_f64(fine._h, coarse._h, _ptr(dst), _ptr(src), _stream()))
+    check(lib().mfx_transfer_prolongate_f64(fine._h, coarse._h, _ptr(dst, F64, fine.m()),
+                                             _ptr(src, F64, coarse.m()), _stream()))
     return dst
 
 
 def transfer_restrict(fine: LaplaceOperator, coarse: LaplaceOperator, src: torch.Tensor):
     dst = coarse.initialize_dof_vector()
-    check(lib().mfx_transfer_restrict_f64(fine._h, coarse._h, _ptr(dst), _ptr(src), _stream()))
+    check(lib().mfx_transfer_restrict_f64(fine._h, coarse._h, _ptr(dst, F64, coarse.m()),
+                                           _ptr(src, F64, fine.m()), _stream()))
     return dst
 
 
 def chebyshev_f32(op: LaplaceOperator, inv_diag: torch.Tensor, x: torch.Tensor, b: torch.Tensor,
                   degree: int, lambda_min: float, lambda_max: float, x_is_zero: bool) -> None:
     """x receives `degree` Chebyshev(Jacobi) steps towards A x = b (FP32)."""
-    check(lib().mfx_chebyshev_f32(op._h, _ptr(inv_diag), _ptr(x), _ptr(b), degree, lambda_min,
+    n = op.m()
+    check(lib().mfx_chebyshev_f32(op._h, _ptr(inv_diag, F32, n), _ptr(x, F32, n), _ptr(b, F32, n), degree, lambda_min,
                                   lambda_max, int(x_is_zero), _stream()))
 
 
@@ -330,7 +344,7 @@ class SolverCG:
         A._check_vec(b)
         res = _lib.CgResult()
         if isinstance(preconditioner, LearnedPreconditioner):
-            st = lib().mfx_nn_fcg_solve(preconditioner._h, A._h, _ptr(x), _ptr(b),
+            st = lib().mfx_nn_fcg_solve(preconditioner._h, A._h, _ptr(x, F64, A.m()), _ptr(b, F64, A.m()),
                                         self.control.max_steps, self.control.tolerance,
                                         self.control.reduction, C.byref(res), _stream())
             self._record(res, st)
@@ -338,19 +352,19 @@ class SolverCG:
         if isinstance(preconditioner, PreconditionMG):
             if A._h.value != preconditioner.operator()._h.value:
                 raise ValueError("A must be the multigrid's fine operator (PreconditionMG.operator())")
-            st = lib().mfx_mg_solve(preconditioner._h, _ptr(x), _ptr(b), self.control.max_steps,
+            st = lib().mfx_mg_solve(preconditioner._h, _ptr(x, F64, A.m()), _ptr(b, F64, A.m()), self.control.max_steps,
                                     self.control.tolerance, self.control.reduction, C.byref(res),
                                     _stream())
             self._record(res, st)
             return
         if isinstance(preconditioner, DiagonalMatrix):
-            inv = _ptr(preconditioner.vector)
+            inv = _ptr(preconditioner.vector, F64, A.m())
         elif isinstance(preconditioner, PreconditionIdentity) or preconditioner is None:
             inv = None
         else:
             raise TypeError("preconditioner must be PreconditionIdentity, DiagonalMatrix or "
                             "PreconditionMG")
-        st = lib().mfx_cg_solve(A._h, _ptr(x), _ptr(b), inv, self.control.max_steps,
+        st = lib().mfx_cg_solve(A._h, _ptr(x, F64, A.m()), _ptr(b, F64, A.m()), inv, self.control.max_steps,
                                 self.control.tolerance, self.control.reduction, C.byref(res),
                                 _stream())
         self._record(res, st)
@@ -410,11 +424,11 @@ class BatchedLaplace:
         return LaplaceOperator(0, 0, _handle=C.c_void_p(h), _owner=self)
 
     def vmult(self, dst: torch.Tensor, src: torch.Tensor) -> None:
-        check(lib().mfx_batch_vmult(self._h, _ptr(dst), _ptr(src), _stream()))
+        check(lib().mfx_batch_vmult(self._h, _ptr(dst, F64, self.m()), _ptr(src, F64, self.m()), _stream()))
 
     def compute_inverse_diagonal(self) -> torch.Tensor:
         d = self.initialize_dof_vector()
-        check(lib().mfx_batch_inverse_diagonal(self._h, _ptr(d), _stream()))
+        check(lib().mfx_batch_inverse_diagonal(self._h, _ptr(d, F64, self.m()), _stream()))
         return d
 
     def zero_constrained(self, v: torch.Tensor) -> None:
@@ -436,8 +450,8 @@ class BatchedLaplace:
         instance; returns one ReductionControl per instance.  Raises
         NoConvergence if any instance fails."""
         res = (_lib.CgResult * self.n_instances)()
-        inv = None if inverse_diagonal is None else _ptr(inverse_diagonal)
-        st = lib().mfx_batch_cg_solve(self._h, _ptr(x), _ptr(b), inv, max_steps, abs_tol, rel_tol,
+        inv = None if inverse_diagonal is None else _ptr(inverse_diagonal, F64, self.m())
+        st = lib().mfx_batch_cg_solve(self._h, _ptr(x, F64, self.m()), _ptr(b, F64, self.m()), inv, max_steps, abs_tol, rel_tol,
                                       int(use_graph), res, _stream())
         out = []
         for r in res:
@@ -523,12 +537,14 @@ class LearnedPreconditioner:
     def forward(self, dims, x: torch.Tensor) -> torch.Tensor:
         """net(x) on a grid of dims = (nx, ny, nz) nodes (FP32 fields)."""
         y = torch.empty_like(x)
-        check(lib().mfx_nn_forward(self._h, self._dims(dims), _ptr(x), _ptr(y), _stream()))
+        n = dims[0] * dims[1] * dims[2]
+        check(lib().mfx_nn_forward(self._h, self._dims(dims), _ptr(x, F32, n), _ptr(y, F32, n), _stream()))
         return y
 
     def gradient(self, dims, x: torch.Tensor, dy: torch.Tensor) -> torch.Tensor:
         g = torch.empty(self.n_params(), dtype=torch.float32, device="cuda")
-        check(lib().mfx_nn_gradient(self._h, self._dims(dims), _ptr(x), _ptr(dy), _ptr(g),
+        n = dims[0] * dims[1] * dims[2]
+        check(lib().mfx_nn_gradient(self._h, self._dims(dims), _ptr(x, F32, n), _ptr(dy, F32, n), _ptr(g, F32),
                                     _stream()))
         return g
 
@@ -537,7 +553,8 @@ class LearnedPreconditioner:
         (x, target: [batch, nz, ny, nx] FP32); returns the loss before it."""
         batch = x.shape[0]
         loss = C.c_double()
-        check(lib().mfx_nn_train_step(self._h, self._dims(dims), batch, _ptr(x), _ptr(target),
+        n = batch * dims[0] * dims[1] * dims[2]
+        check(lib().mfx_nn_train_step(self._h, self._dims(dims), batch, _ptr(x, F32, n), _ptr(target, F32, n),
                                       lr, C.byref(loss), _stream()))
         return loss.value
 
@@ -546,12 +563,13 @@ class LearnedPreconditioner:
         """One Adam step on the energy-norm loss of whole fields (r, e:
         [batch, N] FP64, A e = r); returns the loss before it."""
         loss = C.c_double()
-        check(lib().mfx_nn_train_energy(self._h, op._h, r.shape[0], _ptr(r), _ptr(e), lr,
+        n = r.shape[0] * op.m()
+        check(lib().mfx_nn_train_energy(self._h, op._h, r.shape[0], _ptr(r, F64, n), _ptr(e, F64, n), lr,
                                         C.byref(loss), _stream()))
         return loss.value
 
     def vmult(self, op: LaplaceOperator, z: torch.Tensor, r: torch.Tensor) -> None:
-        check(lib().mfx_nn_vmult(self._h, op._h, _ptr(z), _ptr(r), _stream()))
+        check(lib().mfx_nn_vmult(self._h, op._h, _ptr(z, F64, op.m()), _ptr(r, F64, op.m()), _stream()))
 
 
 def batch_instances(count: int = 64, seed: int = 2, n_min: int = 4, n_max: int = 32,
@@ -582,37 +600,43 @@ def batch_instances(count: int = 64, seed: int = 2, n_min: int = 4, n_max: int =
 def bk5_apply(degree, D, G, src, dst, map_, n_cells):
     """dst += sum_cells P^T D^T G D P src (GLL collocation), no zeroing."""
     fn = lib().mfx_bk5_apply_f64 if src.dtype == torch.float64 else lib().mfx_bk5_apply_f32
-    check(fn(degree, _ptr(D), _ptr(G), _ptr(src), _ptr(dst), _ptr(map_), n_cells, _stream()))
+    t, nm3 = src.dtype, (degree + 1) ** 3
+    check(fn(degree, _ptr(D, t), _ptr(G, t, 6 * nm3 * n_cells), _ptr(src, t), _ptr(dst, t, src.numel()),
+             _ptr(map_, I32, nm3 * n_cells), n_cells, _stream()))
 
 
 def bk3_apply(degree, nq, B, D, G, src, dst, map_, n_cells):
     fn = lib().mfx_bk3_apply_f64 if src.dtype == torch.float64 else lib().mfx_bk3_apply_f32
-    check(fn(degree, nq, _ptr(B), _ptr(D), _ptr(G), _ptr(src), _ptr(dst), _ptr(map_), n_cells,
-             _stream()))
+    t = src.dtype
+    check(fn(degree, nq, _ptr(B, t, nq * (degree + 1)), _ptr(D, t, nq * nq), _ptr(G, t, 6 * nq ** 3 * n_cells),
+             _ptr(src, t), _ptr(dst, t, src.numel()), _ptr(map_, I32, (degree + 1) ** 3 * n_cells),
+             n_cells, _stream()))
 
 
 def evec_apply(degree, nq, collocation, B, D, G, inp, out, n_cells):
     if collocation:
-        check(lib().mfx_bk5_evec_f64(degree, _ptr(D), _ptr(G), _ptr(inp), _ptr(out), n_cells,
+        check(lib().mfx_bk5_evec_f64(degree, _ptr(D, F64), _ptr(G, F64), _ptr(inp, F64), _ptr(out, F64), n_cells,
                                      _stream()))
     else:
-        check(lib().mfx_bk3_evec_f64(degree, nq, _ptr(B), _ptr(D), _ptr(G), _ptr(inp), _ptr(out),
+        check(lib().mfx_bk3_evec_f64(degree, nq, _ptr(B, F64), _ptr(D, F64), _ptr(G, F64), _ptr(inp, F64), _ptr(out, F64),
                                      n_cells, _stream()))
 
 
 def gather(map_, n_cells, nodes_per_cell, src):
     """E-vector of src through the DoF map (invalid entries -> 0)."""
     ev = torch.empty(n_cells * nodes_per_cell, dtype=torch.float64, device="cuda")
-    check(lib().mfx_gather_f64(_ptr(map_), n_cells, nodes_per_cell, _ptr(src), _ptr(ev), _stream()))
+    check(lib().mfx_gather_f64(_ptr(map_, I32, n_cells * nodes_per_cell), n_cells, nodes_per_cell, _ptr(src, F64),
+                                _ptr(ev, F64, n_cells * nodes_per_cell), _stream()))
     return ev
 
 
 def scatter_add(map_, n_cells, nodes_per_cell, evec, dst):
-    check(lib().mfx_scatter_add_f64(_ptr(map_), n_cells, nodes_per_cell, _ptr(evec), _ptr(dst),
+    check(lib().mfx_scatter_add_f64(_ptr(map_, I32, n_cells * nodes_per_cell), n_cells, nodes_per_cell,
+                                    _ptr(evec, F64, n_cells * nodes_per_cell), _ptr(dst, F64),
                                     _stream()))
 
 
 def random_vector(n: int, seed: int) -> torch.Tensor:
     v = torch.empty(n, dtype=torch.float64, device="cuda")
-    check(lib().mfx_vec_random(_ptr(v), n, seed, _stream()))
+    check(lib().mfx_vec_random(_ptr(v, F64, n), n, seed, _stream()))
     return v

@@ -248,6 +248,24 @@ def test_unstructured_create_matches_box(gpu):
     assert _rel(y.cpu().numpy(), P.vmult(x)) <= TOL64
 
 
+def test_vector_type_mismatch_raises_before_launch(gpu):
+    """The C entry points take typed pointers: a vector of the wrong element
+    type (same length) would be read and written past its end, so the Python
+    layer refuses it before any launch."""
+    from benchmarks_amd.laplace import LaplaceOperator
+    op = LaplaceOperator(2, 4, O.GAUSS, (3, 2, 2), with_fp32=True)
+    x32 = torch.zeros(op.m(), dtype=torch.float32, device="cuda")
+    x64 = torch.zeros(op.m(), dtype=torch.float64, device="cuda")
+    with pytest.raises(TypeError):
+        op.vmult_add(x32, x32)
+    with pytest.raises(TypeError):
+        op.vmult(x64, x32)
+    with pytest.raises(TypeError):
+        op.vmult(x32, x64)
+    with pytest.raises(ValueError):
+        op.zero_constrained(x64[:-1])
+
+
 def test_invalid_arguments_raise(gpu):
     from benchmarks_amd._lib import MfxError
     from benchmarks_amd.laplace import LaplaceOperator
