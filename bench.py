@@ -275,7 +275,9 @@ def bench_batch(count=64, seed=2):
     in [4, 32], degree uniform in 2..6, SplitMix64 seed 2), per-instance
     uniform[-1,1] right-hand side, per-instance Jacobi-CG to 1e-8, all in one
     batch (hipGraph of CG steps, autotuned tile per degree), next to the same
-    instances solved one after another with mfx_cg_solve."""
+    instances solved one after another with mfx_cg_solve (each on its
+    operator with stored metric terms and on an on-the-fly one; the speedup is
+    against the faster of the two)."""
     import torch
 
     from benchmarks_amd.laplace import (BatchedLaplace, DiagonalMatrix, ReductionControl,
@@ -299,32 +301,50 @@ def bench_batch(count=64, seed=2):
         ctls = B.solve(x, b, inv, rel_tol=1e-8, use_graph=graph)
         torch.cuda.synchronize()
         res["graph" if graph else "direct"] = (time.perf_counter() - t0, ctls)
-    t_seq, steps_seq = 0.0, []
-    for i in range(len(inst)):
-        a, e = B.offsets[i], B.offsets[i + 1]
-        op = B.instance_operator(i)
-        bi, ii = b[a:e].clone(), inv[a:e].clone()
-        xi = op.initialize_dof_vector()
-        c = ReductionControl(100000, 1e-300, 1e-8)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        SolverCG(c).solve(op, xi, bi, DiagonalMatrix(ii))
-        torch.cuda.synchronize()
-        t_seq += time.perf_counter() - t0
-        steps_seq.append(c.last_step())
+    # the same solves one after another (mfx_cg_solve), on each instance's
+    # operator with stored metric terms and on an on-the-fly one
+    from benchmarks_amd.laplace import LaplaceOperator
+    seq = {}
+    for form in ("stored", "on_the_fly"):
+        t_seq, steps_seq = 0.0, []
+        for i, (p, n) in enumerate(inst):
+            a, e = B.offsets[i], B.offsets[i + 1]
+            op = (B.instance_operator(i) if form == "stored" else
+                  LaplaceOperator(p, p + 1, 1, n, deform=0.05, geometry=ON_THE_FLY))
+            bi, ii = b[a:e].clone(), inv[a:e].clone()
+            xi = op.initialize_dof_vector()
+            SolverCG(ReductionControl(100000, 1e-300, 1e-2)).solve(op, xi, bi, DiagonalMatrix(ii))
+            xi.zero_()
+            c = ReductionControl(100000, 1e-300, 1e-8)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            SolverCG(c).solve(op, xi, bi, DiagonalMatrix(ii))
+            torch.cuda.synchronize()
+            t_seq += time.perf_counter() - t0
+            steps_seq.append(c.last_step())
+        seq[form] = (t_seq, steps_seq)
+    t_seq = min(v[0] for v in seq.values())
+    steps_seq = seq["stored"][1]
     ctls = res["graph"][1]
     tuning = {}
     for p in sorted({p for p, _ in inst}):
         t = B.tuning(p)
         v, d = t["variant"], t["default_variant"]
-        tuning[p] = {"cells": t["n_cells"], "chosen_cells_per_wg": t["cells_per_workgroup"][v],
+        tuning[p] = {"cells": t["n_cells"], "chosen_geometry": t["geometry"][v],
+                     "chosen_cells_per_wg_or_patch": t["cells_per_workgroup"][v],
                      "chosen_prefetch": t["prefetch"][v], "ms_chosen": t["ms"][v],
-                     "ms_default": t["ms"][d]}
+                     "ms_default": t["ms"][d],
+                     "ms_per_variant": {
+                         (f"{g} {c} cells per workgroup, prefetch {pf}" if g == "stored"
+                          else f"{g} patch of {c} cells"): ms
+                         for g, c, pf, ms in zip(t["geometry"], t["cells_per_workgroup"],
+                                                 t["prefetch"], t["ms"])}}
     out = {"workload": f"{count} BP5 instances, cells/dir log-uniform 4..32, degree 2..6, "
                        f"seed {seed}, Jacobi-CG to 1e-8 each",
            "n_dofs": B.m(), "n_cells": sum(n ** 3 for _, n in inst),
            "setup_s": t_setup, "batch_graph_s": res["graph"][0], "batch_direct_s": res["direct"][0],
-           "sequential_s": t_seq, "speedup_vs_sequential": t_seq / res["graph"][0],
+           "sequential_stored_s": seq["stored"][0], "sequential_on_the_fly_s": seq["on_the_fly"][0],
+           "speedup_vs_sequential": t_seq / res["graph"][0],
            "steps_max": max(c.last_step() for c in ctls),
            "steps_total": sum(c.last_step() for c in ctls),
            "steps_differing_from_sequential": sum(1 for c, s in zip(ctls, steps_seq)

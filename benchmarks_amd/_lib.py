@@ -75,7 +75,7 @@ class BatchTuning(C.Structure):
     _fields_ = [("degree", C.c_int), ("n_cells", C.c_uint32), ("n_variants", C.c_int),
                 ("variant", C.c_int), ("default_variant", C.c_int),
                 ("cells_per_workgroup", C.c_int * 8), ("prefetch", C.c_int * 8),
-                ("ms", C.c_double * 8)]
+                ("geometry", C.c_int * 8), ("ms", C.c_double * 8)]
 
 
 _P = C.c_void_p
@@ -128,6 +128,7 @@ _SIGS = {
     "mfx_batch_offsets": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "mfx_batch_instance_operator": (_P, [_P, C.c_int]),
     "mfx_batch_get_tuning": (C.c_int, [_P, C.c_int, C.POINTER(BatchTuning)]),
+    "mfx_batch_set_variant": (C.c_int, [_P, C.c_int, C.c_int]),
     "mfx_batch_vmult": (C.c_int, [_P, _P, _P, _P]),
     "mfx_batch_inverse_diagonal": (C.c_int, [_P, _P, _P]),
     "mfx_batch_cg_solve": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double, C.c_int,

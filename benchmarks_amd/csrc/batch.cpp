@@ -9,9 +9,12 @@
 // scalars and convergence state (batch_kernels.hip); cells and segments of
 // converged instances are skipped.  A solve runs as a hipGraph of K CG steps
 // replayed until every instance has converged (or as the same launches
-// issued directly).  At creation an autotuner may time the tile variants of
-// the collocation kernel (cells per workgroup, G prefetch) per degree on the
-// batch's own cells and keep the fastest.
+// issued directly).  Per degree there is a second form of the apply: the
+// persistent patch kernel with the metric terms computed from the cell
+// vertices (patch.hip), over the concatenated patches of all instances of
+// that degree.  At creation an autotuner may time the tile variants of the
+// collocation kernel (cells per workgroup, G prefetch) and the patch kernel
+// per degree on the batch's own cells and keep the fastest.
 #include <algorithm>
 #include <chrono>
 #include <memory>
@@ -20,6 +23,7 @@
 
 #include "batch.hpp"
 #include "op.hpp"
+#include "patch.hpp"
 
 namespace mfx {
 namespace {
@@ -44,8 +48,17 @@ struct mfx_batch {
     DBuf<uint32_t> map;
     DBuf<double> G, D;
     DBuf<uint16_t> cell_inst;
+    // the on-the-fly form: concatenated patch maps, patch -> (instance,
+    // patch coordinates), instance -> (cells per direction, first cell),
+    // concatenated cell vertices
+    PatchBatchOp patches;
+    DBuf<uint32_t> pmap;
+    DBuf<uint4> meta, inst;
+    DBuf<double> cellX, w, xq;
     int variant = 0, default_variant = 0;
     std::vector<double> ms;  // per variant, when tuned
+    int n_variants() const { return colloc_variant_count(p) + 1; }
+    bool on_the_fly(int v) const { return v == colloc_variant_count(p); }
   };
   std::vector<OpPtr> ops;
   std::vector<uint64_t> off;  // n + 1
@@ -68,10 +81,16 @@ struct mfx_batch {
     if (cs) (void)hipStreamDestroy(cs);
   }
 
-  void apply(const double *src, double *dst, const int *done, hipStream_t s) {
-    for (auto &g : groups)
-      launch_colloc_variant(g.p, g.variant, g.D.ptr, g.G.ptr, src, dst, g.map.ptr, g.E,
+  static void apply_group(const Group &g, int v, const double *src, double *dst, const int *done,
+                          hipStream_t s) {
+    if (g.on_the_fly(v))
+      launch_patch_batch(g.patches, src, dst, done, s);
+    else
+      launch_colloc_variant(g.p, v, g.D.ptr, g.G.ptr, src, dst, g.map.ptr, g.E,
                             done ? g.cell_inst.ptr : nullptr, done, s);
+  }
+  void apply(const double *src, double *dst, const int *done, hipStream_t s) {
+    for (auto &g : groups) apply_group(g, g.variant, src, dst, done, s);
   }
   void vmult(double *dst, const double *src, hipStream_t s) {
     launch_fill(dst, 0.0, N, s);
@@ -110,18 +129,14 @@ void autotune(mfx_batch &b, hipStream_t s) {
     }
   } ev{e0, e1};
   for (auto &g : b.groups) {
-    const int nv = colloc_variant_count(g.p);
+    const int nv = g.n_variants();
     g.ms.assign(nv, 0.0);
     double best = 1e30;
     for (int v = 0; v < nv; ++v) {
-      for (int w = 0; w < 2; ++w)
-        launch_colloc_variant(g.p, v, g.D.ptr, g.G.ptr, src.ptr, dst.ptr, g.map.ptr, g.E, nullptr,
-                              nullptr, s);
+      for (int w = 0; w < 2; ++w) mfx_batch::apply_group(g, v, src.ptr, dst.ptr, nullptr, s);
       MFX_HIP(hipEventRecord(e0, s));
       constexpr int reps = 5;
-      for (int w = 0; w < reps; ++w)
-        launch_colloc_variant(g.p, v, g.D.ptr, g.G.ptr, src.ptr, dst.ptr, g.map.ptr, g.E, nullptr,
-                              nullptr, s);
+      for (int w = 0; w < reps; ++w) mfx_batch::apply_group(g, v, src.ptr, dst.ptr, nullptr, s);
       MFX_HIP(hipEventRecord(e1, s));
       MFX_HIP(hipEventSynchronize(e1));
       float ms = 0.0f;
@@ -133,6 +148,72 @@ void autotune(mfx_batch &b, hipStream_t s) {
       }
     }
   }
+}
+
+// The on-the-fly form of a degree group: every instance's patch map (the
+// same kernel shape for all), offset to its DoF range, concatenated in
+// instance order, with the matching per-patch and per-instance tables.
+void build_patches(mfx_batch &b, mfx_batch::Group &g, int n, hipStream_t s) {
+  const int p = g.p;
+  const PatchShape sh = patch_shape(p);
+  const uint64_t P3 = (uint64_t)(sh.bx * p + 1) * (sh.by * p + 1) * (sh.bz * p + 1);
+  uint64_t np_total = 0;
+  std::vector<uint4> meta, inst(n, uint4{0, 0, 0, 0});
+  uint64_t cells = 0;
+  for (int i = 0; i < n; ++i) {
+    const mfx_op &op = *b.ops[i];
+    if (op.p != p) continue;
+    const uint32_t n0 = op.box.n[0], n1 = op.box.n[1], n2 = op.box.n[2];
+    const uint32_t q0 = (n0 + sh.bx - 1) / sh.bx, q1 = (n1 + sh.by - 1) / sh.by,
+                   q2 = (n2 + sh.bz - 1) / sh.bz;
+    inst[i] = uint4{n0, n1, n2, (uint32_t)cells};
+    for (uint32_t z = 0; z < q2; ++z)
+      for (uint32_t y = 0; y < q1; ++y)
+        for (uint32_t x = 0; x < q0; ++x) meta.push_back(uint4{(uint32_t)i, x, y, z});
+    np_total += (uint64_t)q0 * q1 * q2;
+    cells += op.E;
+  }
+  MFX_REQUIRE(np_total < 0xFFFFFFFFull && np_total * P3 < (1ull << 40) && cells < 0xFFFFFFFFull,
+              "batch too large for the patch tables");
+  g.pmap.alloc(np_total * P3);
+  g.cellX.alloc(cells * 24);
+  DBuf<uint32_t> tmp;
+  uint64_t pt0 = 0;
+  for (int i = 0; i < n; ++i) {
+    const mfx_op &op = *b.ops[i];
+    if (op.p != p) continue;
+    PatchLayout lay{};
+    const int bs[3] = {sh.bx, sh.by, sh.bz};
+    for (int k = 0; k < 3; ++k) {
+      lay.n[k] = op.box.n[k];
+      lay.np[k] = (op.box.n[k] + bs[k] - 1) / bs[k];
+    }
+    const uint64_t npi = patch_count(lay);
+    tmp.alloc(npi * P3);
+    launch_patch_map(lay, p, sh, tmp.ptr, s);
+    launch_offset_map(tmp.ptr, g.pmap.ptr + pt0 * P3, npi * P3, (uint32_t)b.off[i], s);
+    MFX_HIP(hipMemcpyAsync(g.cellX.ptr + (uint64_t)inst[i].w * 24, op.cellX.ptr,
+                           (uint64_t)op.E * 24 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MFX_HIP(hipStreamSynchronize(s));  // tmp is reused
+    pt0 += npi;
+    if (!g.w.ptr) {
+      g.w.alloc(op.wq.n);
+      g.xq.alloc(op.xq.n);
+      MFX_HIP(hipMemcpyAsync(g.w.ptr, op.wq.ptr, op.wq.n * 8, hipMemcpyDeviceToDevice, s));
+      MFX_HIP(hipMemcpyAsync(g.xq.ptr, op.xq.ptr, op.xq.n * 8, hipMemcpyDeviceToDevice, s));
+    }
+  }
+  g.meta.upload(meta.data(), meta.size());
+  g.inst.upload(inst.data(), inst.size());
+  g.patches.p = p;
+  g.patches.n_patches = (uint32_t)np_total;
+  g.patches.grid = patch_batch_grid(p, (uint32_t)np_total);
+  g.patches.pmap = g.pmap.ptr;
+  g.patches.meta = g.meta.ptr;
+  g.patches.inst = g.inst.ptr;
+  g.patches.cellX = g.cellX.ptr;
+  g.patches.w = g.w.ptr;
+  g.patches.xq = g.xq.ptr;
 }
 
 }  // namespace
@@ -200,6 +281,7 @@ mfx_status mfx_batch_create(const mfx_batch_instance *inst, int n, int tune, mfx
         }
         e0 += op.E;
       }
+      build_patches(*b, g, n, s);
       g.default_variant = colloc_default_variant(p);
       g.variant = g.default_variant;
       b->groups.push_back(std::move(g));
@@ -264,14 +346,36 @@ mfx_status mfx_batch_get_tuning(const mfx_batch *b, int degree, mfx_batch_tuning
     for (auto &g : b->groups) {
       if (g.p != degree) continue;
       t->n_cells = g.E;
-      t->n_variants = colloc_variant_count(degree);
+      t->n_variants = g.n_variants();
       t->variant = g.variant;
       t->default_variant = g.default_variant;
       for (int v = 0; v < t->n_variants && v < 8; ++v) {
-        launch_colloc_variant(degree, v, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr,
-                              nullptr, nullptr, &t->cells_per_workgroup[v], &t->prefetch[v]);
+        if (g.on_the_fly(v)) {
+          const PatchShape sh = patch_shape(degree);
+          t->cells_per_workgroup[v] = sh.bx * sh.by * sh.bz;
+          t->prefetch[v] = 1;
+          t->geometry[v] = MFX_GEOM_ON_THE_FLY;
+        } else {
+          launch_colloc_variant(degree, v, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                                nullptr, nullptr, nullptr, &t->cells_per_workgroup[v],
+                                &t->prefetch[v]);
+          t->geometry[v] = MFX_GEOM_STORED;
+        }
         t->ms[v] = v < (int)g.ms.size() ? g.ms[v] : 0.0;
       }
+      return;
+    }
+    throw Error{MFX_ERR_INVALID_ARG, "no instance of degree " + std::to_string(degree)};
+  });
+}
+
+mfx_status mfx_batch_set_variant(mfx_batch *b, int degree, int variant) {
+  return guard([&] {
+    MFX_REQUIRE(b, "null pointer argument");
+    for (auto &g : b->groups) {
+      if (g.p != degree) continue;
+      MFX_REQUIRE(variant >= 0 && variant < g.n_variants(), "variant out of range");
+      g.variant = variant;
       return;
     }
     throw Error{MFX_ERR_INVALID_ARG, "no instance of degree " + std::to_string(degree)};

@@ -37,6 +37,27 @@ struct PatchOp {
   const uint32_t *pmap = nullptr;
 };
 
+// Patches of many box operators of one degree in one launch (batched
+// solves): the patch maps of the instances concatenated in patch order, with
+// the instances' DoF offsets applied; per patch its instance and patch
+// coordinates; per instance its cell counts and first cell in the
+// concatenated vertex array.  Patches of instances whose `done` flag is set
+// are skipped.
+struct PatchBatchOp {
+  int p = 0;
+  uint32_t n_patches = 0;
+  uint32_t grid = 0;
+  const uint32_t *pmap = nullptr;  // [n_patches][P3]
+  const uint4 *meta = nullptr;     // per patch: instance, px, py, pz
+  const uint4 *inst = nullptr;     // per instance: n0, n1, n2, first cell
+  const double *cellX = nullptr;   // [cells][8][3]
+  const double *w = nullptr, *xq = nullptr;
+};
+uint32_t patch_batch_grid(int p, uint32_t n_patches);
+// dst += A src over the batch's patches (FP64, no zeroing)
+void launch_patch_batch(const PatchBatchOp &o, const double *src, double *dst, const int *done,
+                        hipStream_t s);
+
 PatchShape patch_shape(int p);
 uint32_t patch_count(const PatchLayout &lay);
 template <typename T>

@@ -439,10 +439,16 @@ class BatchedLaplace:
         t = _lib.BatchTuning()
         check(lib().mfx_batch_get_tuning(self._h, degree, C.byref(t)))
         nv = t.n_variants
+        geo = {_lib.GEOM_STORED: "stored", _lib.GEOM_ON_THE_FLY: "on_the_fly"}
         return {"degree": t.degree, "n_cells": t.n_cells, "variant": t.variant,
                 "default_variant": t.default_variant,
                 "cells_per_workgroup": list(t.cells_per_workgroup)[:nv],
-                "prefetch": list(t.prefetch)[:nv], "ms": list(t.ms)[:nv]}
+                "prefetch": list(t.prefetch)[:nv],
+                "geometry": [geo[g] for g in list(t.geometry)[:nv]], "ms": list(t.ms)[:nv]}
+
+    def set_variant(self, degree: int, variant: int) -> None:
+        """Use tuning variant `variant` for the instances of `degree`."""
+        check(lib().mfx_batch_set_variant(self._h, degree, variant))
 
     def solve(self, x: torch.Tensor, b: torch.Tensor, inverse_diagonal=None, max_steps=100000,
               abs_tol=1e-300, rel_tol=1e-8, use_graph=True):

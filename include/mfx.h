@@ -332,13 +332,17 @@ typedef struct {
   int n_variants;              /* tile variants of the collocation kernel */
   int variant;                 /* the one in use */
   int default_variant;         /* the library's default tile */
-  int cells_per_workgroup[8];  /* per variant */
+  int cells_per_workgroup[8];  /* per variant (cells per patch for the patch kernel) */
   int prefetch[8];             /* per variant: metric terms loaded ahead */
+  int geometry[8];             /* per variant: MFX_GEOM_STORED (collocation kernel tiles) or
+                                  MFX_GEOM_ON_THE_FLY (the persistent patch kernel, last) */
   double ms[8];                /* per variant: measured apply time (0 if not tuned) */
 } mfx_batch_tuning;
 
-/* autotune != 0: time every tile variant per degree on the batch's cells and
- * keep the fastest; otherwise the default tile */
+/* autotune != 0: time every variant per degree (the collocation kernel's
+ * tiles with stored metric terms, and the patch kernel with metric terms
+ * from the cell vertices) on the batch's cells and keep the fastest;
+ * otherwise the default tile */
 mfx_status mfx_batch_create(const mfx_batch_instance *instances, int n_instances,
                             int autotune, mfx_batch **out);
 mfx_status mfx_batch_destroy(mfx_batch *batch);
@@ -348,6 +352,9 @@ mfx_status mfx_batch_offsets(const mfx_batch *batch, uint64_t *offsets /* n + 1 
 /* the instance's own operator (owned by the batch) */
 mfx_op *mfx_batch_instance_operator(mfx_batch *batch, int instance);
 mfx_status mfx_batch_get_tuning(const mfx_batch *batch, int degree, mfx_batch_tuning *tuning);
+/* use variant `variant` (0 .. n_variants - 1 of the tuning record) for the
+ * instances of `degree` */
+mfx_status mfx_batch_set_variant(mfx_batch *batch, int degree, int variant);
 mfx_status mfx_batch_vmult(mfx_batch *batch, double *dst, const double *src, void *stream);
 mfx_status mfx_batch_inverse_diagonal(mfx_batch *batch, double *inv, void *stream);
 /* inv_diag NULL: identity preconditioner.  results: one per instance.

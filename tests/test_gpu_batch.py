@@ -1,9 +1,10 @@
 """Batched solves (mfx_batch_*): many BP5 problems of uneven size and degree
 on one concatenated vector.  Parity: the batched operator equals every
-instance's own operator (<= 1e-13 relative; only the atomic summation order
-differs), per-instance Jacobi-CG takes the oracle's number of steps (the
+instance's own operator (<= 1e-13 relative with the same geometry form, only
+the atomic summation order differs; <= 1e-12 between stored and on-the-fly
+metric terms), per-instance Jacobi-CG takes the oracle's number of steps (the
 knife-edge rule of test_gpu_cg), the hipGraph path equals the direct one, and
-the autotuner keeps the fastest measured tile."""
+the autotuner keeps the fastest measured variant."""
 import numpy as np
 import pytest
 
@@ -48,6 +49,70 @@ def test_batch_vmult_equals_instance_operators(gpu):
         yi = op.initialize_dof_vector()
         op.vmult(yi, x[a:b].clone())
         assert (torch.linalg.norm(y[a:b] - yi) / torch.linalg.norm(yi)).item() <= 1e-13, i
+
+
+@pytest.mark.parametrize("form", ["stored", "on_the_fly"])
+def test_every_variant_equals_instance_operators(gpu, form):
+    """Each variant of each degree (the collocation tiles with stored metric
+    terms, and the batched patch kernel computing them from the vertices)
+    applies every instance's own operator; ragged instances leave partial
+    patches.  <= 1e-12: the two geometry forms round the metric differently."""
+    import torch
+    from benchmarks_amd.laplace import random_vector
+    B = _batch(SMALL)
+    x = random_vector(B.m(), 11)
+    ref = []
+    for i in range(len(SMALL)):
+        a, b = B.offsets[i], B.offsets[i + 1]
+        op = B.instance_operator(i)
+        yi = op.initialize_dof_vector()
+        op.vmult(yi, x[a:b].clone())
+        ref.append(yi)
+    degrees = sorted({p for p, _ in SMALL})
+    nv = {p: len(B.tuning(p)["ms"]) for p in degrees}
+    geo = {p: B.tuning(p)["geometry"] for p in degrees}
+    for k in range(max(nv.values())):
+        chosen = {}
+        for p in degrees:
+            vs = [v for v in range(nv[p]) if geo[p][v] == form]
+            chosen[p] = vs[min(k, len(vs) - 1)]
+            B.set_variant(p, chosen[p])
+        y = B.initialize_dof_vector()
+        B.vmult(y, x)
+        for i in range(len(SMALL)):
+            a, b = B.offsets[i], B.offsets[i + 1]
+            err = (torch.linalg.norm(y[a:b] - ref[i]) / torch.linalg.norm(ref[i])).item()
+            assert err <= 1e-12, (form, chosen, i, err)
+
+
+def test_batch_cg_on_the_fly_matches_oracle(gpu):
+    """Per-instance Jacobi-CG with the batched patch kernel: the oracle's
+    steps (knife-edge rule) and solution."""
+    B = _batch(SMALL)
+    for p in sorted({p for p, _ in SMALL}):
+        t = B.tuning(p)
+        B.set_variant(p, t["geometry"].index("on_the_fly"))
+    b = _rhs(B, 6)
+    inv = B.compute_inverse_diagonal()
+    x = B.initialize_dof_vector()
+    ctls = B.solve(x, b, inv, rel_tol=1e-8)
+    bh, xh = b.cpu().numpy(), x.cpu().numpy()
+    for i, (p, n) in enumerate(SMALL):
+        a, e = B.offsets[i], B.offsets[i + 1]
+        P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
+        xo, r, hist = P.cg(bh[a:e], inv_diag=P.inverse_diagonal(), rel_tol=1e-8, abs_tol=1e-300)
+        assert r.converged
+        assert_same_steps(ctls[i].last_step(), r.steps, hist, 1e-8)
+        assert np.linalg.norm(xh[a:e] - xo) <= 1e-7 * np.linalg.norm(xo), i
+
+
+def test_batch_set_variant_rejects_bad_input(gpu):
+    from benchmarks_amd._lib import MfxError
+    B = _batch([(3, 2), (4, 2)])
+    with pytest.raises(MfxError):
+        B.set_variant(3, 7)
+    with pytest.raises(MfxError):
+        B.set_variant(5, 0)
 
 
 def test_batch_vmult_matches_oracle(gpu):
@@ -116,10 +181,14 @@ def test_batch_cg_matches_single_solves(gpu):
         assert (torch.linalg.norm(x[a:e] - xi) / torch.linalg.norm(xi)).item() <= 1e-6
 
 
-def test_batch_converged_instance_is_left_alone(gpu):
+@pytest.mark.parametrize("form", ["stored", "on_the_fly"])
+def test_batch_converged_instance_is_left_alone(gpu, form):
     """An instance with a zero right-hand side is done before the first step;
-    its solution stays zero while the others iterate."""
+    its solution stays zero while the others iterate (its cells, or its
+    patches, are skipped)."""
     B = _batch([(3, 4), (2, 6), (4, 3)])
+    for p in (2, 3, 4):
+        B.set_variant(p, B.tuning(p)["geometry"].index(form))
     b = _rhs(B)
     a, e = B.offsets[1], B.offsets[2]
     b[a:e] = 0
@@ -137,7 +206,8 @@ def test_autotuner_keeps_fastest_variant(gpu):
     U = _batch(inst, autotune=False)
     for p in range(2, 7):
         t = B.tuning(p)
-        assert len(t["ms"]) == 6 and min(t["ms"]) > 0
+        assert len(t["ms"]) == 7 and min(t["ms"]) > 0
+        assert t["geometry"] == ["stored"] * 6 + ["on_the_fly"]
         assert t["ms"][t["variant"]] == min(t["ms"])
         u = U.tuning(p)
         assert u["variant"] == u["default_variant"] and max(u["ms"]) == 0
