@@ -211,6 +211,14 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
 
   for (int i = tid; i < NQ * NM; i += Tile<NQ>::THREADS) sB[i] = gB[i];
   for (int i = tid; i < NQ * NQ; i += Tile<NQ>::THREADS) sD[i] = gD[i];
+  // wave-uniform entries (the z contractions): through the scalar cache up
+  // to nq = 7, from LDS above (their scalar registers would spill)
+  auto uB = [&](int i, int j) -> T {
+    if constexpr (NQ <= 7) return gB[i * NM + j]; else return sB[i * NM + j];
+  };
+  auto uD = [&](int i, int j) -> T {
+    if constexpr (NQ <= 7) return gD[i * NQ + j]; else return sD[i * NQ + j];
+  };
 
   // 1. gather the (x,y) node column
   uint32_t idx[NM];
@@ -235,11 +243,14 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
 
   // 2. interpolate along x: s1[z][y][qx], z,y < NM
   if (active && qy < NM) {
+    T bx[NM];  // this lane's row of B, read from LDS once
+#pragma unroll
+    for (int a = 0; a < NM; ++a) bx[a] = sB[qx * NM + a];
 #pragma unroll
     for (int z = 0; z < NM; ++z) {
       T t = 0;
 #pragma unroll
-      for (int a = 0; a < NM; ++a) t += sB[qx * NM + a] * s0[el][z * NM2 + qy * NM + a];
+      for (int a = 0; a < NM; ++a) t += bx[a] * s0[el][z * NM2 + qy * NM + a];
       s1[el][z * NM * NQ + qy * NQ + qx] = t;
     }
   }
@@ -248,19 +259,21 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   // 3./4. interpolate along y (LDS) and z (registers), then 5. gradient
   T uq[NQ];
   if (active) {
-    T t2[NM];
+    T t2[NM], by[NM];
+#pragma unroll
+    for (int b = 0; b < NM; ++b) by[b] = sB[qy * NM + b];
 #pragma unroll
     for (int z = 0; z < NM; ++z) {
       T t = 0;
 #pragma unroll
-      for (int b = 0; b < NM; ++b) t += sB[qy * NM + b] * s1[el][z * NM * NQ + b * NQ + qx];
+      for (int b = 0; b < NM; ++b) t += by[b] * s1[el][z * NM * NQ + b * NQ + qx];
       t2[z] = t;
     }
 #pragma unroll
     for (int zq = 0; zq < NQ; ++zq) {
       T t = 0;
 #pragma unroll
-      for (int c = 0; c < NM; ++c) t += sB[zq * NM + c] * t2[c];
+      for (int c = 0; c < NM; ++c) t += uB(zq, c) * t2[c];
       uq[zq] = t;
       s2[el][zq * NQ2 + col] = t;
     }
@@ -269,6 +282,12 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
 
   T fz[NQ];
   if (active) {
+    T dxr[NQ], dyr[NQ];  // this lane's rows of D
+#pragma unroll
+    for (int a = 0; a < NQ; ++a) {
+      dxr[a] = sD[qx * NQ + a];
+      dyr[a] = sD[qy * NQ + a];
+    }
     const T *Ge = G + (size_t)e * 6 * NQ3 + col;
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
@@ -281,9 +300,9 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       T dx = 0, dy = 0, dz = 0;
 #pragma unroll
       for (int a = 0; a < NQ; ++a) {
-        dx += sD[qx * NQ + a] * s2[el][z * NQ2 + qy * NQ + a];
-        dy += sD[qy * NQ + a] * s2[el][z * NQ2 + a * NQ + qx];
-        dz += sD[z * NQ + a] * uq[a];
+        dx += dxr[a] * s2[el][z * NQ2 + qy * NQ + a];
+        dy += dyr[a] * s2[el][z * NQ2 + a * NQ + qx];
+        dz += uD(z, a) * uq[a];
       }
       s0[el][z * NQ2 + col] = gxx * dx + gxy * dy + gxz * dz;
       s1[el][z * NQ2 + col] = gxy * dx + gyy * dy + gyz * dz;
@@ -294,15 +313,20 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
 
   // 6. transposed gradient, 7. transposed interpolation along z (registers)
   if (active) {
-    T w[NQ];
+    T w[NQ], dxt[NQ], dyt[NQ];  // this lane's columns of D
+#pragma unroll
+    for (int a = 0; a < NQ; ++a) {
+      dxt[a] = sD[a * NQ + qx];
+      dyt[a] = sD[a * NQ + qy];
+    }
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
       T v = 0;
 #pragma unroll
       for (int a = 0; a < NQ; ++a) {
-        v += sD[a * NQ + qx] * s0[el][z * NQ2 + qy * NQ + a];
-        v += sD[a * NQ + qy] * s1[el][z * NQ2 + a * NQ + qx];
-        v += sD[a * NQ + z] * fz[a];
+        v += dxt[a] * s0[el][z * NQ2 + qy * NQ + a];
+        v += dyt[a] * s1[el][z * NQ2 + a * NQ + qx];
+        v += uD(a, z) * fz[a];
       }
       w[z] = v;
     }
@@ -310,7 +334,7 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
     for (int zn = 0; zn < NM; ++zn) {
       T t = 0;
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) t += sB[c * NM + zn] * w[c];
+      for (int c = 0; c < NQ; ++c) t += uB(c, zn) * w[c];
       s2[el][zn * NQ2 + col] = t;
     }
   }
@@ -318,11 +342,14 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
 
   // 8. transposed interpolation along y: s0[zn][yn][qx]
   if (active && qy < NM) {
+    T byt[NQ];
+#pragma unroll
+    for (int b = 0; b < NQ; ++b) byt[b] = sB[b * NM + qy];
 #pragma unroll
     for (int zn = 0; zn < NM; ++zn) {
       T t = 0;
 #pragma unroll
-      for (int b = 0; b < NQ; ++b) t += sB[b * NM + qy] * s2[el][zn * NQ2 + b * NQ + qx];
+      for (int b = 0; b < NQ; ++b) t += byt[b] * s2[el][zn * NQ2 + b * NQ + qx];
       s0[el][zn * NM * NQ + qy * NQ + qx] = t;
     }
   }
@@ -331,11 +358,14 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   // 9. transposed interpolation along x and scatter
   double dot = 0.0;
   if (node_col) {
+    T bxt[NQ];
+#pragma unroll
+    for (int a = 0; a < NQ; ++a) bxt[a] = sB[a * NM + qx];
 #pragma unroll
     for (int zn = 0; zn < NM; ++zn) {
       T v = 0;
 #pragma unroll
-      for (int a = 0; a < NQ; ++a) v += sB[a * NM + qx] * s0[el][zn * NM * NQ + qy * NQ + a];
+      for (int a = 0; a < NQ; ++a) v += bxt[a] * s0[el][zn * NM * NQ + qy * NQ + a];
       if (DOT) dot += (double)uin[zn] * (double)v;
       if (GATHER) {
         if (idx[zn] != kInvalid) atomic_add(dst + idx[zn], v);
