@@ -143,7 +143,8 @@ class LaplaceOperator:
         return torch.zeros(self.m(), dtype=dtype, device="cuda")
 
     def vmult(self, dst: torch.Tensor, src: torch.Tensor) -> None:
-        """dst = A src (constrained rows: identity), CEED_bp/include/portable_laplace_operator.h:124-172."""
+        """dst = A src (constrained rows: identity),
+        CEED_bp/include/portable_laplace_operator.h:124-172."""
         self._check_vec(dst)
         self._check_vec(src)
         if dst.dtype == torch.float64:
@@ -263,7 +264,8 @@ class PreconditionMG:
             n_cells = (n_cells,) * 3
         d = _lib.MgDesc()
         f = d.fine
-        f.degree, f.n_q_points_1d, f.quadrature, f.geometry = degree, n_q_points_1d, quadrature, geometry
+        f.degree, f.n_q_points_1d = degree, n_q_points_1d
+        f.quadrature, f.geometry = quadrature, geometry
         for k in range(3):
             f.n_cells[k], f.lo[k], f.hi[k] = n_cells[k], lo[k], hi[k]
         f.deform = deform
@@ -327,7 +329,8 @@ def chebyshev_f32(op: LaplaceOperator, inv_diag: torch.Tensor, x: torch.Tensor, 
                   degree: int, lambda_min: float, lambda_max: float, x_is_zero: bool) -> None:
     """x receives `degree` Chebyshev(Jacobi) steps towards A x = b (FP32)."""
     n = op.m()
-    check(lib().mfx_chebyshev_f32(op._h, _ptr(inv_diag, F32, n), _ptr(x, F32, n), _ptr(b, F32, n), degree, lambda_min,
+    check(lib().mfx_chebyshev_f32(op._h, _ptr(inv_diag, F32, n), _ptr(x, F32, n), _ptr(b, F32, n),
+                                  degree, lambda_min,
                                   lambda_max, int(x_is_zero), _stream()))
 
 
@@ -344,15 +347,18 @@ class SolverCG:
         A._check_vec(b)
         res = _lib.CgResult()
         if isinstance(preconditioner, LearnedPreconditioner):
-            st = lib().mfx_nn_fcg_solve(preconditioner._h, A._h, _ptr(x, F64, A.m()), _ptr(b, F64, A.m()),
+            st = lib().mfx_nn_fcg_solve(preconditioner._h, A._h, _ptr(x, F64, A.m()),
+                                        _ptr(b, F64, A.m()),
                                         self.control.max_steps, self.control.tolerance,
                                         self.control.reduction, C.byref(res), _stream())
             self._record(res, st)
             return
         if isinstance(preconditioner, PreconditionMG):
             if A._h.value != preconditioner.operator()._h.value:
-                raise ValueError("A must be the multigrid's fine operator (PreconditionMG.operator())")
-            st = lib().mfx_mg_solve(preconditioner._h, _ptr(x, F64, A.m()), _ptr(b, F64, A.m()), self.control.max_steps,
+                raise ValueError("A must be the multigrid's fine operator "
+                                 "(PreconditionMG.operator())")
+            st = lib().mfx_mg_solve(preconditioner._h, _ptr(x, F64, A.m()), _ptr(b, F64, A.m()),
+                                    self.control.max_steps,
                                     self.control.tolerance, self.control.reduction, C.byref(res),
                                     _stream())
             self._record(res, st)
@@ -364,7 +370,8 @@ class SolverCG:
         else:
             raise TypeError("preconditioner must be PreconditionIdentity, DiagonalMatrix or "
                             "PreconditionMG")
-        st = lib().mfx_cg_solve(A._h, _ptr(x, F64, A.m()), _ptr(b, F64, A.m()), inv, self.control.max_steps,
+        st = lib().mfx_cg_solve(A._h, _ptr(x, F64, A.m()), _ptr(b, F64, A.m()), inv,
+                                self.control.max_steps,
                                 self.control.tolerance, self.control.reduction, C.byref(res),
                                 _stream())
         self._record(res, st)
@@ -424,7 +431,8 @@ class BatchedLaplace:
         return LaplaceOperator(0, 0, _handle=C.c_void_p(h), _owner=self)
 
     def vmult(self, dst: torch.Tensor, src: torch.Tensor) -> None:
-        check(lib().mfx_batch_vmult(self._h, _ptr(dst, F64, self.m()), _ptr(src, F64, self.m()), _stream()))
+        check(lib().mfx_batch_vmult(self._h, _ptr(dst, F64, self.m()), _ptr(src, F64, self.m()),
+                                    _stream()))
 
     def compute_inverse_diagonal(self) -> torch.Tensor:
         d = self.initialize_dof_vector()
@@ -457,7 +465,8 @@ class BatchedLaplace:
         NoConvergence if any instance fails."""
         res = (_lib.CgResult * self.n_instances)()
         inv = None if inverse_diagonal is None else _ptr(inverse_diagonal, F64, self.m())
-        st = lib().mfx_batch_cg_solve(self._h, _ptr(x, F64, self.m()), _ptr(b, F64, self.m()), inv, max_steps, abs_tol, rel_tol,
+        st = lib().mfx_batch_cg_solve(self._h, _ptr(x, F64, self.m()), _ptr(b, F64, self.m()), inv,
+                                      max_steps, abs_tol, rel_tol,
                                       int(use_graph), res, _stream())
         out = []
         for r in res:
@@ -544,13 +553,15 @@ class LearnedPreconditioner:
         """net(x) on a grid of dims = (nx, ny, nz) nodes (FP32 fields)."""
         y = torch.empty_like(x)
         n = dims[0] * dims[1] * dims[2]
-        check(lib().mfx_nn_forward(self._h, self._dims(dims), _ptr(x, F32, n), _ptr(y, F32, n), _stream()))
+        check(lib().mfx_nn_forward(self._h, self._dims(dims), _ptr(x, F32, n), _ptr(y, F32, n),
+                                   _stream()))
         return y
 
     def gradient(self, dims, x: torch.Tensor, dy: torch.Tensor) -> torch.Tensor:
         g = torch.empty(self.n_params(), dtype=torch.float32, device="cuda")
         n = dims[0] * dims[1] * dims[2]
-        check(lib().mfx_nn_gradient(self._h, self._dims(dims), _ptr(x, F32, n), _ptr(dy, F32, n), _ptr(g, F32),
+        check(lib().mfx_nn_gradient(self._h, self._dims(dims), _ptr(x, F32, n), _ptr(dy, F32, n),
+                                    _ptr(g, F32),
                                     _stream()))
         return g
 
@@ -560,7 +571,8 @@ class LearnedPreconditioner:
         batch = x.shape[0]
         loss = C.c_double()
         n = batch * dims[0] * dims[1] * dims[2]
-        check(lib().mfx_nn_train_step(self._h, self._dims(dims), batch, _ptr(x, F32, n), _ptr(target, F32, n),
+        check(lib().mfx_nn_train_step(self._h, self._dims(dims), batch, _ptr(x, F32, n),
+                                      _ptr(target, F32, n),
                                       lr, C.byref(loss), _stream()))
         return loss.value
 
@@ -570,12 +582,14 @@ class LearnedPreconditioner:
         [batch, N] FP64, A e = r); returns the loss before it."""
         loss = C.c_double()
         n = r.shape[0] * op.m()
-        check(lib().mfx_nn_train_energy(self._h, op._h, r.shape[0], _ptr(r, F64, n), _ptr(e, F64, n), lr,
+        check(lib().mfx_nn_train_energy(self._h, op._h, r.shape[0], _ptr(r, F64, n),
+                                        _ptr(e, F64, n), lr,
                                         C.byref(loss), _stream()))
         return loss.value
 
     def vmult(self, op: LaplaceOperator, z: torch.Tensor, r: torch.Tensor) -> None:
-        check(lib().mfx_nn_vmult(self._h, op._h, _ptr(z, F64, op.m()), _ptr(r, F64, op.m()), _stream()))
+        check(lib().mfx_nn_vmult(self._h, op._h, _ptr(z, F64, op.m()), _ptr(r, F64, op.m()),
+                                 _stream()))
 
 
 def batch_instances(count: int = 64, seed: int = 2, n_min: int = 4, n_max: int = 32,
@@ -607,37 +621,43 @@ def bk5_apply(degree, D, G, src, dst, map_, n_cells):
     """dst += sum_cells P^T D^T G D P src (GLL collocation), no zeroing."""
     fn = lib().mfx_bk5_apply_f64 if src.dtype == torch.float64 else lib().mfx_bk5_apply_f32
     t, nm3 = src.dtype, (degree + 1) ** 3
-    check(fn(degree, _ptr(D, t), _ptr(G, t, 6 * nm3 * n_cells), _ptr(src, t), _ptr(dst, t, src.numel()),
+    check(fn(degree, _ptr(D, t), _ptr(G, t, 6 * nm3 * n_cells), _ptr(src, t),
+             _ptr(dst, t, src.numel()),
              _ptr(map_, I32, nm3 * n_cells), n_cells, _stream()))
 
 
 def bk3_apply(degree, nq, B, D, G, src, dst, map_, n_cells):
     fn = lib().mfx_bk3_apply_f64 if src.dtype == torch.float64 else lib().mfx_bk3_apply_f32
     t = src.dtype
-    check(fn(degree, nq, _ptr(B, t, nq * (degree + 1)), _ptr(D, t, nq * nq), _ptr(G, t, 6 * nq ** 3 * n_cells),
+    check(fn(degree, nq, _ptr(B, t, nq * (degree + 1)), _ptr(D, t, nq * nq),
+             _ptr(G, t, 6 * nq ** 3 * n_cells),
              _ptr(src, t), _ptr(dst, t, src.numel()), _ptr(map_, I32, (degree + 1) ** 3 * n_cells),
              n_cells, _stream()))
 
 
 def evec_apply(degree, nq, collocation, B, D, G, inp, out, n_cells):
     if collocation:
-        check(lib().mfx_bk5_evec_f64(degree, _ptr(D, F64), _ptr(G, F64), _ptr(inp, F64), _ptr(out, F64), n_cells,
+        check(lib().mfx_bk5_evec_f64(degree, _ptr(D, F64), _ptr(G, F64), _ptr(inp, F64),
+                                     _ptr(out, F64), n_cells,
                                      _stream()))
     else:
-        check(lib().mfx_bk3_evec_f64(degree, nq, _ptr(B, F64), _ptr(D, F64), _ptr(G, F64), _ptr(inp, F64), _ptr(out, F64),
+        check(lib().mfx_bk3_evec_f64(degree, nq, _ptr(B, F64), _ptr(D, F64), _ptr(G, F64),
+                                     _ptr(inp, F64), _ptr(out, F64),
                                      n_cells, _stream()))
 
 
 def gather(map_, n_cells, nodes_per_cell, src):
     """E-vector of src through the DoF map (invalid entries -> 0)."""
     ev = torch.empty(n_cells * nodes_per_cell, dtype=torch.float64, device="cuda")
-    check(lib().mfx_gather_f64(_ptr(map_, I32, n_cells * nodes_per_cell), n_cells, nodes_per_cell, _ptr(src, F64),
+    check(lib().mfx_gather_f64(_ptr(map_, I32, n_cells * nodes_per_cell), n_cells, nodes_per_cell,
+                               _ptr(src, F64),
                                 _ptr(ev, F64, n_cells * nodes_per_cell), _stream()))
     return ev
 
 
 def scatter_add(map_, n_cells, nodes_per_cell, evec, dst):
-    check(lib().mfx_scatter_add_f64(_ptr(map_, I32, n_cells * nodes_per_cell), n_cells, nodes_per_cell,
+    check(lib().mfx_scatter_add_f64(_ptr(map_, I32, n_cells * nodes_per_cell), n_cells,
+                                    nodes_per_cell,
                                     _ptr(evec, F64, n_cells * nodes_per_cell), _ptr(dst, F64),
                                     _stream()))
 
