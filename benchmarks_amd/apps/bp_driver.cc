@@ -17,6 +17,15 @@
 //                                                    to 1e-10 preconditioned by the FP32
 //                                                    multigrid V-cycle, next to
 //                                                    Jacobi-CG to 1e-8
+//   bp_driver --batch <count> [seed]                 batched BP5 solves: count instances,
+//                                                    cells per direction log-uniform in
+//                                                    [4, 32], degree uniform in 2..6
+//                                                    (SplitMix64 seed, default 2),
+//                                                    per-instance Jacobi-CG to 1e-8
+//   bp_driver --fcg <degree> <n> <smoother_degree>   BP5 on n^3 deformed cells, flexible
+//                                                    CG to 1e-8 with the learned
+//                                                    preconditioner's Chebyshev cycle
+//                                                    (network off), next to Jacobi-CG
 //   bp_driver --verify <bk3|bk5> <nq> <nelmt>        the verification fixture of
 //       CEED_BK/src/BK3/serial_verification.cc:25-79 and
 //       sum_factorization/tests/BK5/serial_verification.cc:16-47 (G = 2, in = 3,
@@ -26,6 +35,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -192,6 +202,95 @@ static int run_mg(unsigned degree, unsigned n, unsigned nq) {
   return 0;
 }
 
+// u in [0, 1) from the SplitMix64 stream of `seed` (k-th draw: z = seed +
+// k 0x9E3779B97F4A7C15, standard mix, (z >> 11) 2^-53)
+struct SplitMix {
+  std::uint64_t seed, k = 0;
+  double operator()() {
+    std::uint64_t z = seed + (++k) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return double(z >> 11) * 0x1.0p-53;
+  }
+};
+
+static int run_batch(unsigned count, std::uint64_t seed) {
+  SplitMix u{seed};
+  std::vector<mfx::BatchInstance> inst(count);
+  for (auto &in : inst) {
+    const double t = u();
+    const unsigned n = (unsigned)std::lround(std::exp(std::log(4.0) + t * (std::log(32.0) - std::log(4.0))));
+    in.degree = 2 + std::min(unsigned(u() * 5), 4u);
+    for (int k = 0; k < 3; ++k) in.mesh.n_cells[k] = n;
+    in.mesh.deform = 0.05;
+  }
+  auto t0 = Clock::now();
+  mfx::BatchedLaplace batch(inst, true);
+  check_hip_sync();
+  const double t_setup = seconds_since(t0);
+  VectorD b, x, inv;
+  batch.initialize_dof_vector(b);
+  batch.initialize_dof_vector(x);
+  mfx::check(mfx_vec_random(b.get_values(), b.size(), seed, nullptr));
+  const auto off = batch.offsets();
+  for (unsigned i = 0; i < count; ++i)
+    mfx::check(mfx_vec_zero_constrained(mfx_batch_instance_operator(batch.handle(), (int)i),
+                                        b.get_values() + off[i], nullptr));
+  batch.compute_inverse_diagonal(inv);
+  batch.solve(x, b, &inv, 100000, 1e-300, 1e-2);  // loads the kernels
+  x = 0.;
+  check_hip_sync();
+  t0 = Clock::now();
+  const auto res = batch.solve(x, b, &inv, 100000, 1e-300, 1e-8);
+  const double t_solve = seconds_since(t0);
+  int steps_max = 0, steps_total = 0, converged = 0;
+  for (const auto &r : res) {
+    steps_max = std::max(steps_max, r.steps);
+    steps_total += r.steps;
+    converged += r.converged;
+  }
+  std::printf("variants:");
+  for (unsigned p = 2; p <= 6; ++p) {
+    mfx_batch_tuning t{};
+    if (mfx_batch_get_tuning(batch.handle(), (int)p, &t) != MFX_OK) continue;
+    std::printf(" p%u:%s", p, t.geometry[t.variant] == MFX_GEOM_ON_THE_FLY ? "on_the_fly" : "stored");
+  }
+  std::printf("\nRESULT batch instances=%u dofs=%llu setup_s=%.6e solve_s=%.6e steps_max=%d "
+              "steps_total=%d converged=%d\n",
+              count, (unsigned long long)batch.m(), t_setup, t_solve, steps_max, steps_total,
+              converged);
+  return converged == (int)count ? 0 : 1;
+}
+
+static int run_fcg(unsigned degree, unsigned n, int smoother_degree) {
+  mfx::BoxMesh mesh;
+  for (int k = 0; k < 3; ++k) mesh.n_cells[k] = n;
+  mesh.deform = 0.05;
+  mfx::LaplaceOperator<double> op(mesh, degree, degree + 1, mfx::Quadrature::gauss_lobatto, true,
+                                  mfx::Geometry::on_the_fly);
+  VectorD b, x;
+  op.initialize_dof_vector(b);
+  op.initialize_dof_vector(x);
+  mfx::check(mfx_vec_random(b.get_values(), b.size(), 1, nullptr));
+  mfx::check(mfx_vec_zero_constrained(op.handle(), b.get_values(), nullptr));
+  mfx::LearnedPreconditioner P(1);
+  P.set_mode(smoother_degree, false);
+  mfx::ReductionControl cf(5000, 1e-300, 1e-8);
+  mfx::SolverCG<VectorD> fcg(cf);
+  fcg.solve(op, x, b, P);
+  mfx::DiagonalMatrix<VectorD> jacobi;
+  op.compute_inverse_diagonal(jacobi.get_vector());
+  mfx::ReductionControl cj(100000, 1e-300, 1e-8);
+  mfx::SolverCG<VectorD> cg(cj);
+  x = 0.;
+  cg.solve(op, x, b, jacobi);
+  std::printf("RESULT fcg p=%u cells=%u dofs=%llu smoother_degree=%d fcg_its=%u jacobi_its=%u\n",
+              degree, op.n_cells(), (unsigned long long)op.m(), smoother_degree, cf.last_step(),
+              cj.last_step());
+  return 0;
+}
+
 // E-vector kernels on the reference's verification fixture.
 static int run_verify(const std::string &kind, unsigned nq, unsigned nelmt) {
   const bool bk5 = kind == "bk5";
@@ -234,6 +333,20 @@ int main(int argc, char **argv) {
       }
       const unsigned degree = std::atoi(argv[2]);
       return run_mg(degree, std::atoi(argv[3]), argc > 4 ? std::atoi(argv[4]) : degree + 2);
+    }
+    if (argc > 1 && std::string(argv[1]) == "--batch") {
+      if (argc < 3) {
+        std::printf("Usage: bp_driver --batch count [seed]\n");
+        return 1;
+      }
+      return run_batch(std::atoi(argv[2]), argc > 3 ? std::atoll(argv[3]) : 2);
+    }
+    if (argc > 1 && std::string(argv[1]) == "--fcg") {
+      if (argc < 5) {
+        std::printf("Usage: bp_driver --fcg degree n smoother_degree\n");
+        return 1;
+      }
+      return run_fcg(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]));
     }
     if (argc > 1 && std::string(argv[1]) == "--bp5") {
       if (argc < 4) {

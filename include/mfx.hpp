@@ -11,7 +11,11 @@
 //   SolverControl::NoConvergence (CEED_bp/src/bp3.cc:264-288,
 //   bp5_kokkos/benchmark.cc:336-378);
 //   PreconditionMG: the mixed-precision p/h-multigrid of mfx_mg_* (no
-//   counterpart in the reference; deal.II's PreconditionChebyshev parameters).
+//   counterpart in the reference; deal.II's PreconditionChebyshev parameters);
+//   BatchedLaplace: many independent problems solved together (mfx_batch_*);
+//   LearnedPreconditioner: the network preconditioner of mfx_nn_*, with
+//   SolverCG::solve running flexible CG for it.
+// None of these three has a counterpart in the reference.
 // Errors become exceptions (mfx::Error / NoConvergence), as deal.II's do.
 #pragma once
 
@@ -125,6 +129,9 @@ struct BoxMesh {
 };
 
 enum class Quadrature { gauss = MFX_QUAD_GAUSS, gauss_lobatto = MFX_QUAD_GLL };
+// metric terms stored per quadrature point, or computed on the fly from the
+// cell vertices (GLL collocation only)
+enum class Geometry { stored = MFX_GEOM_STORED, on_the_fly = MFX_GEOM_ON_THE_FLY };
 
 template <typename Number = double>
 class LaplaceOperator {
@@ -134,12 +141,13 @@ class LaplaceOperator {
   using VectorType = Vector<Number>;
 
   LaplaceOperator(const BoxMesh &mesh, unsigned fe_degree, unsigned n_q_points_1d,
-                  Quadrature quad = Quadrature::gauss, bool with_fp32 = false) {
+                  Quadrature quad = Quadrature::gauss, bool with_fp32 = false,
+                  Geometry geometry = Geometry::stored) {
     mfx_box_desc d{};
     d.degree = static_cast<int>(fe_degree);
     d.n_q_points_1d = static_cast<int>(n_q_points_1d);
     d.quadrature = static_cast<int>(quad);
-    d.geometry = MFX_GEOM_STORED;
+    d.geometry = static_cast<int>(geometry);
     for (int k = 0; k < 3; ++k) {
       d.n_cells[k] = mesh.n_cells[k];
       d.lo[k] = mesh.p1[k];
@@ -255,6 +263,126 @@ class PreconditionMG {
   std::vector<LaplaceOperator<double>> fine_;
 };
 
+// Independent BP5 problems (GLL collocation, degree 2..6) on one
+// concatenated vector; per-instance Jacobi or identity CG in one batch.
+struct BatchInstance {
+  unsigned degree = 2;
+  BoxMesh mesh;
+};
+
+class BatchedLaplace {
+ public:
+  explicit BatchedLaplace(const std::vector<BatchInstance> &instances, bool autotune = true) {
+    std::vector<mfx_batch_instance> d(instances.size());
+    for (std::size_t i = 0; i < instances.size(); ++i) {
+      d[i].degree = static_cast<int>(instances[i].degree);
+      for (int k = 0; k < 3; ++k) {
+        d[i].n_cells[k] = instances[i].mesh.n_cells[k];
+        d[i].lo[k] = instances[i].mesh.p1[k];
+        d[i].hi[k] = instances[i].mesh.p2[k];
+      }
+      d[i].deform = instances[i].mesh.deform;
+    }
+    check(mfx_batch_create(d.data(), static_cast<int>(d.size()), autotune ? 1 : 0, &b_));
+  }
+  BatchedLaplace(const BatchedLaplace &) = delete;
+  BatchedLaplace &operator=(const BatchedLaplace &) = delete;
+  ~BatchedLaplace() {
+    if (b_) mfx_batch_destroy(b_);
+  }
+  unsigned n_instances() const { return static_cast<unsigned>(mfx_batch_n_instances(b_)); }
+  std::uint64_t m() const { return mfx_batch_n_dofs(b_); }
+  // DoF range [offsets[i], offsets[i + 1]) of instance i
+  std::vector<std::uint64_t> offsets() const {
+    std::vector<std::uint64_t> o(n_instances() + 1);
+    check(mfx_batch_offsets(b_, o.data()));
+    return o;
+  }
+  void initialize_dof_vector(Vector<double> &v) const { v.reinit(m()); }
+  void vmult(Vector<double> &dst, const Vector<double> &src) const {
+    check(mfx_batch_vmult(b_, dst.get_values(), src.get_values(), nullptr));
+  }
+  void compute_inverse_diagonal(Vector<double> &inv) const {
+    inv.reinit(m(), true);
+    check(mfx_batch_inverse_diagonal(b_, inv.get_values(), nullptr));
+  }
+  mfx_batch_tuning tuning(unsigned degree) const {
+    mfx_batch_tuning t{};
+    check(mfx_batch_get_tuning(b_, static_cast<int>(degree), &t));
+    return t;
+  }
+  void set_variant(unsigned degree, int variant) {
+    check(mfx_batch_set_variant(b_, static_cast<int>(degree), variant));
+  }
+  // per-instance CG (Jacobi when inv_diag is given) with the stopping rule
+  // of ReductionControl(max_steps, tolerance, reduction); one result per
+  // instance; MFX_ERR_NOT_CONVERGED if any instance fails
+  std::vector<mfx_cg_result> solve(Vector<double> &x, const Vector<double> &b,
+                                   const Vector<double> *inv_diag, unsigned max_steps,
+                                   double tolerance, double reduction,
+                                   bool use_graph = true) const {
+    std::vector<mfx_cg_result> r(n_instances());
+    check(mfx_batch_cg_solve(b_, x.get_values(), b.get_values(),
+                             inv_diag ? inv_diag->get_values() : nullptr,
+                             static_cast<int>(max_steps), tolerance, reduction,
+                             use_graph ? 1 : 0, r.data(), nullptr));
+    return r;
+  }
+  mfx_batch *handle() const { return b_; }
+
+ private:
+  mfx_batch *b_ = nullptr;
+};
+
+// The learned preconditioner: with smoother degree k > 0, k Chebyshev steps,
+// the network's correction (on a coarse operator's grid when one is set)
+// with the energy-optimal weight, k Chebyshev steps; nonlinear, so solve
+// with SolverCG::solve(A, x, b, P), which runs flexible CG.
+class LearnedPreconditioner {
+ public:
+  explicit LearnedPreconditioner(std::uint64_t seed = 0) { check(mfx_nn_create(seed, &nn_)); }
+  LearnedPreconditioner(const LearnedPreconditioner &) = delete;
+  LearnedPreconditioner &operator=(const LearnedPreconditioner &) = delete;
+  ~LearnedPreconditioner() {
+    if (nn_) mfx_nn_destroy(nn_);
+  }
+  static std::uint64_t n_params() { return mfx_nn_n_params(); }
+  std::vector<float> get_params() const {
+    std::vector<float> w(n_params());
+    check(mfx_nn_get_params(nn_, w.data()));
+    return w;
+  }
+  void set_params(const std::vector<float> &w) {
+    if (w.size() != n_params()) throw Error(MFX_ERR_INVALID_ARG, "wrong number of parameters");
+    check(mfx_nn_set_params(nn_, w.data()));
+  }
+  void set_output_scale(double s) { check(mfx_nn_set_output_scale(nn_, s)); }
+  double output_scale() const { return mfx_nn_output_scale(nn_); }
+  void set_mode(int smoother_degree, bool use_network = true) {
+    check(mfx_nn_set_mode(nn_, smoother_degree, use_network ? 1 : 0));
+  }
+  // the operator whose node grid the network runs on; nullptr: the fine grid
+  void set_coarse(const LaplaceOperator<double> *coarse) {
+    check(mfx_nn_set_coarse(nn_, coarse ? coarse->handle() : nullptr));
+  }
+  // one Adam step on the energy-norm loss over `batch` consecutive fields
+  // (r, e: batch x A.m(), A e = r); returns the loss before the step
+  double train_energy_step(const LaplaceOperator<double> &A, const Vector<double> &r,
+                           const Vector<double> &e, int batch, double lr) {
+    double loss = 0.0;
+    check(mfx_nn_train_energy(nn_, A.handle(), batch, r.get_values(), e.get_values(), lr, &loss,
+                              nullptr));
+    return loss;
+  }
+  void vmult(const LaplaceOperator<double> &A, Vector<double> &z, const Vector<double> &r) const {
+    check(mfx_nn_vmult(nn_, A.handle(), z.get_values(), r.get_values(), nullptr));
+  }
+  mfx_nn *handle() const { return nn_; }
+
+ private:
+  mfx_nn *nn_ = nullptr;
+};
+
 class SolverControl {
  public:
   struct NoConvergence : public std::runtime_error {
@@ -326,6 +454,18 @@ class SolverCG {
   void solve(const MatrixType &A, VectorType &x, const VectorType &b,
              const DiagonalMatrix<VectorType> &P) {
     run(A, x, b, P.get_vector().get_values());
+  }
+  // flexible CG (4 stored directions) preconditioned by the network
+  void solve(const LaplaceOperator<double> &A, VectorType &x, const VectorType &b,
+             const LearnedPreconditioner &P) {
+    mfx_cg_result r{};
+    const mfx_status s = mfx_nn_fcg_solve(P.handle(), A.handle(), x.get_values(), b.get_values(),
+                                          static_cast<int>(control_.max_steps()),
+                                          control_.tolerance(), control_.reduction(), &r, nullptr);
+    control_.record(r);
+    if (s == MFX_ERR_NOT_CONVERGED || s == MFX_ERR_BREAKDOWN)
+      throw SolverControl::NoConvergence(control_.last_step(), control_.last_value());
+    check(s);
   }
   // FP64 CG preconditioned by one multigrid V-cycle per step
   template <typename MatrixType>

@@ -63,3 +63,26 @@ def test_driver_mixed_precision_multigrid(gpu):
     assert mg_its <= 12 and j_its >= 10 * mg_its
     assert diff <= 1e-6
     assert "levels: Q4/16^3 Q2/16^3 Q1/16^3 Q1/8^3" in text
+
+
+def test_driver_batch_matches_python_instances(gpu):
+    """BatchedLaplace through the C++ API on the batched-solve instance
+    generator (C++ SplitMix64, the stream of laplace.batch_instances): the
+    same DoF count as the Python list, and every instance converges."""
+    from benchmarks_amd.laplace import batch_instances
+    inst = batch_instances(12, 2)
+    dofs = sum((n * p + 1) ** 3 for p, n in inst)
+    text = run("--batch", 12, 2)
+    m = re.search(r"RESULT batch instances=12 dofs=(\d+) .* steps_max=(\d+) .* converged=(\d+)",
+                  text)
+    assert m, text
+    assert int(m.group(1)) == dofs and int(m.group(3)) == 12 and int(m.group(2)) > 10
+
+
+def test_driver_flexible_cg(gpu):
+    """LearnedPreconditioner and SolverCG's flexible CG through the C++ API
+    (network off: the Chebyshev cycle alone) beat Jacobi-CG in steps."""
+    text = run("--fcg", 3, 6, 2)
+    m = re.search(r"RESULT fcg p=3 cells=216 .* fcg_its=(\d+) jacobi_its=(\d+)", text)
+    assert m, text
+    assert 0 < int(m.group(1)) < int(m.group(2))
