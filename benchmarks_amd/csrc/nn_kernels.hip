@@ -289,6 +289,149 @@ k_conv_wgrad(const uint16_t *__restrict__ in, const float *__restrict__ in32,
   }
 }
 
+// The same weight gradient on MFMA (bf16 inputs, FP32 accumulation), for
+// channel counts that are multiples of 16: per tap t an implicit GEMM
+// dW_t[o][c] = sum_p G[p][o] X[src(p, t)][c] with M = output channels,
+// N = input channels and K = grid points, 32 points per
+// v_mfma_f32_16x16x32_bf16.  Each wave owns every (o, c) tile and a
+// strided share of the part's 32-point chunks; the output gradients G are
+// rounded to bf16 (the activations X are stored in bf16 already).  The four
+// waves are summed in LDS into the same per-part partials as k_conv_wgrad.
+template <int CIN, int COUT, int STRIDE, bool UP>
+__global__ void __launch_bounds__(kBlock)
+k_conv_wgrad_mfma(const uint16_t *__restrict__ in, const float *__restrict__ G,
+                  float *__restrict__ part, Dims od, Dims id, uint32_t n_parts) {
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts must be multiples of 16");
+  constexpr int OT = COUT / 16, CT = CIN / 16, NW = kBlock / 64;
+  __shared__ float red[NW][OT * CT * 4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = blockIdx.x;
+  const uint32_t pi = blockIdx.y;
+  const uint64_t P = (uint64_t)od.x * od.y * od.z;
+  const uint64_t p0 = P * pi / n_parts, p1 = P * (pi + 1) / n_parts;
+  const int kq = 8 * (lane >> 4), li = lane & 15;
+  v4f acc[OT][CT];
+#pragma unroll
+  for (int a = 0; a < OT; ++a)
+#pragma unroll
+    for (int b = 0; b < CT; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (uint64_t cb = p0 + 32 * (uint64_t)wave; cb < p1; cb += 32 * NW) {
+    // this lane's 8 points of the chunk: k = kq .. kq + 7
+    int64_t src[8];
+    uint64_t pt[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      pt[q] = cb + kq + q;
+      src[q] = -1;
+      if (pt[q] < p1) {
+        int x, y, z;
+        decode(pt[q], od, x, y, z);
+        src[q] = tap_source<STRIDE, UP>(x, y, z, t, od, id);
+      }
+    }
+    v8bf bx[CT], ag[OT];
+#pragma unroll
+    for (int b = 0; b < CT; ++b) {
+      uint16_t h[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        h[q] = src[q] >= 0 ? in[(uint64_t)src[q] * CIN + b * 16 + li] : (uint16_t)0;
+      const uint4 raw = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16,
+                                   h[4] | (uint32_t)h[5] << 16, h[6] | (uint32_t)h[7] << 16);
+      bx[b] = __builtin_bit_cast(v8bf, raw);
+    }
+#pragma unroll
+    for (int a = 0; a < OT; ++a) {
+      uint16_t h[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) h[q] = pt[q] < p1 ? f2bf(G[pt[q] * COUT + a * 16 + li]) : 0;
+      const uint4 raw = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16,
+                                   h[4] | (uint32_t)h[5] << 16, h[6] | (uint32_t)h[7] << 16);
+      ag[a] = __builtin_bit_cast(v8bf, raw);
+    }
+#pragma unroll
+    for (int a = 0; a < OT; ++a)
+#pragma unroll
+      for (int b = 0; b < CT; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ag[a], bx[b], acc[a][b], 0, 0, 0);
+  }
+#pragma unroll
+  for (int a = 0; a < OT; ++a)
+#pragma unroll
+    for (int b = 0; b < CT; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[wave][(a * CT + b) * 4 + r][lane] = acc[a][b][r];
+  __syncthreads();
+  // D layout: register r of lane l holds (o, c) = (16 a + 4 (l / 16) + r, 16 b + l % 16)
+  for (int e = threadIdx.x; e < OT * CT * 4 * 64; e += kBlock) {
+    const int tr = e / 64, l = e % 64;
+    float v = 0.f;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) v += red[w][tr][l];
+    const int tile = tr / 4, r = tr % 4, a = tile / CT, b = tile % CT;
+    const int o = a * 16 + 4 * (l >> 4) + r, c = b * 16 + (l & 15);
+    part[((uint64_t)pi * COUT + o) * 27 * CIN + (uint64_t)t * CIN + c] = v;
+  }
+}
+
+// Weight gradient of the two 1-channel layers (stride 1, no up-sampling):
+// the input layer (1 -> 16, FP32 input in32) and the head (16 -> 1, bf16
+// input in, output gradient with one channel).  Per tap only 16 weights, so
+// every thread accumulates all 16 over its share of the part's points; the
+// sums go through wave shuffles and LDS into the same per-part partials.
+template <bool HEAD>
+__global__ void __launch_bounds__(kBlock)
+k_conv_wgrad_thin(const uint16_t *__restrict__ in, const float *__restrict__ in32,
+                  const float *__restrict__ G, float *__restrict__ part, Dims od,
+                  uint32_t n_parts) {
+  __shared__ float red[kBlock / 64][16];
+  const int t = blockIdx.x;
+  const uint32_t pi = blockIdx.y;
+  const uint64_t P = (uint64_t)od.x * od.y * od.z;
+  const uint64_t p0 = P * pi / n_parts, p1 = P * (pi + 1) / n_parts;
+  float acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = 0.f;
+  for (uint64_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
+    int x, y, z;
+    decode(p, od, x, y, z);
+    const int64_t src = tap_source<1, false>(x, y, z, t, od, od);
+    if (src < 0) continue;
+    if constexpr (HEAD) {  // dW[0][t][c] = sum_p dy[p] act[p + off_t][c]
+      const float g = G[p];
+      const uint16_t *a = in + (uint64_t)src * 16;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) acc[k] += g * bf2f(a[k]);
+    } else {  // dW[o][t][0] = sum_p G[p][o] x[p + off_t]
+      const float xv = in32[src];
+      const float *g = G + p * 16;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) acc[k] += g[k] * xv;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    float v = acc[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    acc[k] = v;
+  }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < 16; ++k) red[threadIdx.x >> 6][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int k = threadIdx.x;
+    float v = 0.f;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) v += red[w][k];
+    // HEAD: cout 1, cin 16 (c = k); else cout 16 (o = k), cin 1
+    const uint64_t idx = HEAD ? ((uint64_t)pi * 27 + t) * 16 + k
+                              : ((uint64_t)pi * 16 + k) * 27 + t;
+    part[idx] = v;
+  }
+}
+
 // grad[w] += sum over parts of part[part][w] (fixed order)
 __global__ void k_wgrad_reduce(const float *__restrict__ part, float *__restrict__ grad, uint32_t nw,
                                uint32_t n_parts) {
@@ -448,20 +591,52 @@ void nn_add_mask(const float *a, const float *b, const uint16_t *act, float *out
   MFX_HIP(hipGetLastError());
 }
 
+// the MFMA weight gradient for the layer shapes of the U-Net; false if the
+// shape has no MFMA instance (then the VALU kernel runs)
+template <int CI, int CO, int ST, bool UPF>
+bool wgrad_mfma_if(int stride, bool up, const uint16_t *in, const float *G, float *part,
+                   const uint32_t *od, const uint32_t *id, int cin, int cout, uint32_t n_parts,
+                   hipStream_t s) {
+  if (!(in && cin == CI && cout == CO && stride == ST && up == UPF)) return false;
+  hipLaunchKernelGGL((k_conv_wgrad_mfma<CI, CO, ST, UPF>), dim3(27, n_parts), dim3(kBlock), 0, s,
+                     in, G, part, D3(od), D3(id), n_parts);
+  return true;
+}
+bool wgrad_mfma(int stride, bool up, const uint16_t *in, const float *G, float *part,
+                const uint32_t *od, const uint32_t *id, int cin, int cout, uint32_t n_parts,
+                hipStream_t s) {
+  return wgrad_mfma_if<32, 16, 1, true>(stride, up, in, G, part, od, id, cin, cout, n_parts, s) ||
+         wgrad_mfma_if<64, 32, 1, true>(stride, up, in, G, part, od, id, cin, cout, n_parts, s) ||
+         wgrad_mfma_if<64, 64, 1, true>(stride, up, in, G, part, od, id, cin, cout, n_parts, s) ||
+         wgrad_mfma_if<64, 64, 2, false>(stride, up, in, G, part, od, id, cin, cout, n_parts, s) ||
+         wgrad_mfma_if<32, 64, 2, false>(stride, up, in, G, part, od, id, cin, cout, n_parts, s) ||
+         wgrad_mfma_if<16, 32, 2, false>(stride, up, in, G, part, od, id, cin, cout, n_parts, s);
+}
+
 void nn_conv_wgrad(int stride, bool up, const uint16_t *in, const float *in32, const float *G,
                    float *part, float *grad, const uint32_t *od, const uint32_t *id, int cin,
                    int cout, uint32_t n_parts, hipStream_t s) {
   MFX_REQUIRE(cin <= 64 && cout <= 64 && cin * cout <= 16 * kBlock, "weight tile too large");
   const dim3 grid(27, n_parts);
-  if (stride == 2)
-    hipLaunchKernelGGL((k_conv_wgrad<2, false>), grid, dim3(kBlock), 0, s, in, in32, G, part,
-                       D3(od), D3(id), cin, cout, n_parts);
-  else if (up)
-    hipLaunchKernelGGL((k_conv_wgrad<1, true>), grid, dim3(kBlock), 0, s, in, in32, G, part,
-                       D3(od), D3(id), cin, cout, n_parts);
-  else
-    hipLaunchKernelGGL((k_conv_wgrad<1, false>), grid, dim3(kBlock), 0, s, in, in32, G, part,
-                       D3(od), D3(id), cin, cout, n_parts);
+  // the U-Net's layers with 16-multiple channel counts run on MFMA, the two
+  // 1-channel layers on the thin kernel
+  if (stride == 1 && !up && cin == 16 && cout == 1 && in) {
+    hipLaunchKernelGGL(k_conv_wgrad_thin<true>, grid, dim3(kBlock), 0, s, in, nullptr, G, part,
+                       D3(od), n_parts);
+  } else if (stride == 1 && !up && cin == 1 && cout == 16 && in32) {
+    hipLaunchKernelGGL(k_conv_wgrad_thin<false>, grid, dim3(kBlock), 0, s, nullptr, in32, G,
+                       part, D3(od), n_parts);
+  } else if (!wgrad_mfma(stride, up, in, G, part, od, id, cin, cout, n_parts, s)) {
+    if (stride == 2)
+      hipLaunchKernelGGL((k_conv_wgrad<2, false>), grid, dim3(kBlock), 0, s, in, in32, G, part,
+                         D3(od), D3(id), cin, cout, n_parts);
+    else if (up)
+      hipLaunchKernelGGL((k_conv_wgrad<1, true>), grid, dim3(kBlock), 0, s, in, in32, G, part,
+                         D3(od), D3(id), cin, cout, n_parts);
+    else
+      hipLaunchKernelGGL((k_conv_wgrad<1, false>), grid, dim3(kBlock), 0, s, in, in32, G, part,
+                         D3(od), D3(id), cin, cout, n_parts);
+  }
   const uint32_t nw = 27u * cin * cout;
   hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks(nw)), dim3(kBlock), 0, s, part, grad, nw, n_parts);
   MFX_HIP(hipGetLastError());
