@@ -220,6 +220,14 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
     if constexpr (NQ <= 7) return gD[i * NQ + j]; else return sD[i * NQ + j];
   };
 
+  // nq >= 9 (two cells per workgroup): the metric terms are loaded into
+  // registers after the x interpolation, so their stream overlaps the y/z
+  // interpolation (3-20% faster at p = 7, 8 with q = 9, 10; slower at nq = 7
+  // in FP64, no gain below).  Smaller nq loads them in the gradient phase.
+  constexpr bool PF = NQ >= 9;
+  const T *Ge = G + (size_t)(active ? e : 0) * 6 * NQ3 + col;
+  T g[PF ? 6 : 1][PF ? NQ : 1];
+
   // 1. gather the (x,y) node column
   uint32_t idx[NM];
   T uin[NM];
@@ -258,6 +266,12 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
 
   // 3./4. interpolate along y (LDS) and z (registers), then 5. gradient
   T uq[NQ];
+  if (PF && active) {
+#pragma unroll
+    for (int z = 0; z < NQ; ++z)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) g[c][z] = __builtin_nontemporal_load(Ge + c * NQ3 + z * NQ2);
+  }
   if (active) {
     T t2[NM], by[NM];
 #pragma unroll
@@ -288,15 +302,20 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       dxr[a] = sD[qx * NQ + a];
       dyr[a] = sD[qy * NQ + a];
     }
-    const T *Ge = G + (size_t)e * 6 * NQ3 + col;
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
-      const T gxx = __builtin_nontemporal_load(Ge + 0 * NQ3 + z * NQ2);
-      const T gxy = __builtin_nontemporal_load(Ge + 1 * NQ3 + z * NQ2);
-      const T gxz = __builtin_nontemporal_load(Ge + 2 * NQ3 + z * NQ2);
-      const T gyy = __builtin_nontemporal_load(Ge + 3 * NQ3 + z * NQ2);
-      const T gyz = __builtin_nontemporal_load(Ge + 4 * NQ3 + z * NQ2);
-      const T gzz = __builtin_nontemporal_load(Ge + 5 * NQ3 + z * NQ2);
+      T gxx, gxy, gxz, gyy, gyz, gzz;
+      if constexpr (PF) {
+        gxx = g[0][z]; gxy = g[1][z]; gxz = g[2][z];
+        gyy = g[3][z]; gyz = g[4][z]; gzz = g[5][z];
+      } else {
+        gxx = __builtin_nontemporal_load(Ge + 0 * NQ3 + z * NQ2);
+        gxy = __builtin_nontemporal_load(Ge + 1 * NQ3 + z * NQ2);
+        gxz = __builtin_nontemporal_load(Ge + 2 * NQ3 + z * NQ2);
+        gyy = __builtin_nontemporal_load(Ge + 3 * NQ3 + z * NQ2);
+        gyz = __builtin_nontemporal_load(Ge + 4 * NQ3 + z * NQ2);
+        gzz = __builtin_nontemporal_load(Ge + 5 * NQ3 + z * NQ2);
+      }
       T dx = 0, dy = 0, dz = 0;
 #pragma unroll
       for (int a = 0; a < NQ; ++a) {
