@@ -6,7 +6,8 @@
 // solver of the same BP problems:
 //   levels   fine Q_p -> Q_{p/2} -> ... -> Q_1 on the same cells (p-levels),
 //            then Q_1 on n/2, n/4, ... cells per direction (h-levels) until
-//            the free DoFs fit a dense direct solve;
+//            the free DoFs fit a dense direct solve; under a Gauss fine
+//            operator the coarser levels use Gauss q = p+1;
 //   smoother Chebyshev iteration with the Jacobi preconditioner, eigenvalue
 //            interval [lambda_max / range, lambda_max] with lambda_max from a
 //            few CG (Lanczos) steps times a safety factor (the semantics of
@@ -351,13 +352,17 @@ mfx_status mfx_mg_create(const mfx_mg_desc *d, mfx_mg **out) {
     std::vector<mfx_box_desc> descs;
     mfx_box_desc cur = d->fine;
     cur.with_fp32 = 1;
-    const int q_extra = d->fine.n_q_points_1d - d->fine.degree;
     descs.push_back(cur);
+    // levels below the fine one: Gauss q = p+1 under a Gauss fine operator
+    // (exact for the stiffness of affine cells; the fine level keeps its own
+    // rule), otherwise the fine operator's q - p
+    const int q_coarse = cur.quadrature == MFX_QUAD_GAUSS ? 1 : cur.n_q_points_1d - cur.degree;
     while (cur.degree > 1) {
       cur.degree = std::max(1, cur.degree / 2);
-      cur.n_q_points_1d = cur.degree + q_extra;
+      cur.n_q_points_1d = cur.degree + q_coarse;
       descs.push_back(cur);
     }
+    cur.n_q_points_1d = cur.degree + q_coarse;
     auto free_dofs = [](const mfx_box_desc &b) {
       uint64_t f = 1;
       for (int k = 0; k < 3; ++k) f *= (uint64_t)b.n_cells[k] * b.degree - 1;

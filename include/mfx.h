@@ -252,7 +252,8 @@ mfx_status mfx_vec_dot(const double *x, const double *y, uint64_t n,
  *
  * Levels: Q_p, Q_{p/2}, ..., Q_1 on the fine cells, then Q_1 on n/2, n/4, ...
  * cells per direction while the free DoFs exceed coarse_max_dofs (every
- * halving needs even cell counts).  The coarsest level is solved directly in
+ * halving needs even cell counts).  Under a Gauss fine operator the levels
+ * below it use Gauss q = p+1; a GLL hierarchy keeps the fine q - p.  The coarsest level is solved directly in
  * FP64; every other level smooths with Chebyshev(Jacobi) in FP32.  Transfers
  * are nodal interpolation P and its transpose R = P^T.
  * ---------------------------------------------------------------------- */

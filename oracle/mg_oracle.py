@@ -119,9 +119,11 @@ def lanczos_lambda_max(vmult, dinv, r0, iters):
 def level_descriptors(p, nq, quad, n, coarse_max_dofs):
     """(degree, nq, quad, cells) of each level: degrees p, p/2, ..., 1 on the
     fine cells, then Q1 on halved meshes while the free DoFs exceed the
-    direct-solve limit and every cell count is even."""
+    direct-solve limit and every cell count is even.  Below the fine level a
+    Gauss hierarchy uses q = p+1 points (exact for the stiffness of affine
+    cells); a GLL one keeps q - p of the fine level."""
     n = tuple(n) if not isinstance(n, int) else (n, n, n)
-    extra = nq - p
+    extra = 1 if quad == O.GAUSS else nq - p
     out = [(p, nq, quad, n)]
     while p > 1:
         p = max(1, p // 2)

@@ -113,7 +113,9 @@ def test_lanczos_estimate_meets_spectrum():
 def test_level_descriptors():
     d = M.level_descriptors(4, 6, O.GAUSS, 16, 2000)
     assert [(l[0], l[3][0]) for l in d] == [(4, 16), (2, 16), (1, 16), (1, 8)]
-    assert [l[1] for l in d] == [6, 4, 3, 3]
+    assert [l[1] for l in d] == [6, 3, 2, 2]
+    d = M.level_descriptors(1, 3, O.GAUSS, 16, 100)
+    assert [(l[0], l[1], l[3][0]) for l in d] == [(1, 3, 16), (1, 2, 8), (1, 2, 4)]
     d = M.level_descriptors(3, 4, O.GLL, (6, 4, 2), 10)
     assert [(l[0], l[3]) for l in d] == [(3, (6, 4, 2)), (1, (6, 4, 2)), (1, (3, 2, 1))]
 
