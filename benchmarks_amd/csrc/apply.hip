@@ -220,7 +220,7 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
     if constexpr (NQ <= 7) return gD[i * NQ + j]; else return sD[i * NQ + j];
   };
 
-  // nq >= 9 (two cells per workgroup): the metric terms are loaded into
+  // nq >= 9 (two or three cells per workgroup): the metric terms are loaded into
   // registers after the x interpolation, so their stream overlaps the y/z
   // interpolation (3-20% faster at p = 7, 8 with q = 9, 10; slower at nq = 7
   // in FP64, no gain below).  Smaller nq loads them in the gradient phase.
