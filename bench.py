@@ -14,8 +14,10 @@ per-cell kernel) and computed on the fly from the cell vertices (patch
 kernel); the faster mode per degree enters the aggregate.  Also reported
 per degree: the apply kernel's time (HIP events on the launch stream), its
 algorithmic bytes and FLOPs, Jacobi-CG and mixed-precision multigrid CG
-time-to-1e-8 relative residual on the same operator and right-hand side, and
-the mixed-precision BP3 solve (Q4, Gauss q=6, deformed 128^3 cells, 135M
+time-to-1e-8 relative residual on the same operator and right-hand side, the
+BP3 unit-cube case (Q4, Gauss q=6, 8^3 affine cells, 35937 DoFs: vmult
+checksum and time, Jacobi-CG to 1e-10 with the manufactured solution's
+nodal error), the mixed-precision BP3 solve (Q4, Gauss q=6, deformed 128^3 cells, 135M
 DoFs, manufactured-solution right-hand side, FP64 CG to 1e-10 with an FP32
 p/h-multigrid V-cycle) with its error against an FP64 solution, and the
 batched solve of 64 BP5 problems of uneven size and degree (per-instance
@@ -222,6 +224,44 @@ def mg_solve(p, nq, quad, n, geometry, b, x_ref_jacobi, rel_tol):
     del mg, A, x, x_ref
     torch.cuda.empty_cache()
     return res
+
+
+def bench_c1(n=8, p=4, nq=6):
+    """BP3 on the unit cube, n^3 affine cells, Q4, Gauss q=6 (35937 DoFs):
+    one vmult of a SplitMix64 seed-0 vector in [-1, 1] (its 2-norm is the
+    checksum; best of 5 x 200 vmults, as CEED_bp/src/bp3.cc:331-408 times
+    small meshes), then Jacobi-CG to 1e-10 relative with
+    f = 3 pi^2 prod sin(pi x_d), u = 0 on the boundary, x0 = 0: steps, time
+    and the nodal error against the exact solution prod sin(pi x_d)."""
+    import math
+
+    import torch
+
+    from benchmarks_amd.laplace import DiagonalMatrix, LaplaceOperator, random_vector
+    A = LaplaceOperator(p, nq, 0, n_cells=n)
+    x = random_vector(A.m(), 0)
+    y = A.initialize_dof_vector()
+    A.vmult(y, x)
+    e0, e1 = _events()
+    best = float("inf")
+    for _ in range(5):
+        e0.record()
+        for _ in range(200):
+            A.vmult(y, x)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) / 200)
+    b = A.rhs(1)
+    A.zero_constrained(b)
+    u_h = A.initialize_dof_vector()
+    _timed_solve(A, u_h, b, DiagonalMatrix(A.compute_inverse_diagonal()), 1e-10, 100000)
+    t, ctl = _timed_solve(A, u_h, b, DiagonalMatrix(A.compute_inverse_diagonal()), 1e-10, 100000)
+    xyz = A.dof_coordinates()
+    u = torch.sin(math.pi * xyz[:, 0]) * torch.sin(math.pi * xyz[:, 1]) * torch.sin(math.pi * xyz[:, 2])
+    return {"workload": f"BP3 Q{p} Gauss q={nq}, unit cube {n}^3 affine cells, manufactured sin RHS",
+            "n_dofs": A.m(), "vmult_checksum_l2": torch.linalg.norm(y).item(),
+            "ms_vmult_best": best, "cg_jacobi_its_1e-10": ctl.last_step(), "cg_jacobi_s_1e-10": t,
+            "nodal_max_err_vs_exact": ((u_h - u).abs().max() / u.abs().max()).item()}
 
 
 def bench_c3(n=128, p=4, nq=6):
@@ -506,6 +546,7 @@ def main():
     ap.add_argument("--geometry", choices=["both", "stored", "otf"], default="both")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 35937-DoF BP3 unit-cube case")
     ap.add_argument("--no-c3", action="store_true", help="skip the 135M-DoF mixed-precision solve")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched solve")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
@@ -577,6 +618,8 @@ def main():
     traffic = measured_traffic(dom)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
+    if not args.no_cg and not args.no_c1:
+        out["bp3_unit_cube"] = bench_c1()
     if not args.no_cg and not args.no_c3:
         out["mixed_precision_bp3"] = bench_c3()
     if not args.no_cg and not args.no_batch:
