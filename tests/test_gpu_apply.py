@@ -273,6 +273,14 @@ def test_invalid_arguments_raise(gpu):
         LaplaceOperator(9, 10, O.GLL, n_cells=2)
     with pytest.raises(MfxError):
         LaplaceOperator(4, 7, O.GAUSS, n_cells=2)
+    # size limits, checked before anything is allocated: DoF indices are
+    # 31-bit (1291^3 Q1 nodes > 2^31 - 1) and cell counts 32-bit
+    with pytest.raises(MfxError, match="31-bit"):
+        LaplaceOperator(1, 2, O.GLL, n_cells=1290)
+    with pytest.raises(MfxError, match="too many cells"):
+        LaplaceOperator(1, 2, O.GLL, n_cells=(2048, 2048, 512))
+    with pytest.raises(MfxError, match="n_cells"):
+        LaplaceOperator(1, 2, O.GLL, n_cells=(0, 4, 4))
     op = LaplaceOperator(2, 3, O.GLL, n_cells=2)
     x = op.initialize_dof_vector()
     with pytest.raises(MfxError):
