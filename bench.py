@@ -402,21 +402,20 @@ def bench_learned(n=64, p=4, n_fields=32, mse_steps=200, energy_steps=600, degre
     node grid of Q1 on (n/2)^3 cells (restricted residual in, prolongated
     correction out, energy-optimal weight), `degree` Chebyshev steps again.
     The network is trained on (residual, error) pairs of that Q1 operator
-    (Jacobi-CG solves of `n_fields` smooth right-hand sides, 32 random
-    Fourier modes, wave numbers up to n/4, SplitMix64 seed 3): mean squared
-    error, then the energy-norm loss.  Flexible CG to 1e-8 on a fresh smooth
+    (FP64 multigrid-CG solves of `n_fields` smooth right-hand sides, 32
+    random Fourier modes, wave numbers up to n/4, SplitMix64 seed 3): mean
+    squared error, then the energy-norm loss.  Flexible CG to 1e-8 on a fresh smooth
     right-hand side, next to the same cycle without the network, Jacobi-CG
     and multigrid CG; errors against an FP64 solution certified to 1e-12."""
     import torch
 
-    from benchmarks_amd.laplace import (DiagonalMatrix, LaplaceOperator, LearnedPreconditioner,
-                                        PreconditionMG)
+    from benchmarks_amd.laplace import DiagonalMatrix, LearnedPreconditioner, PreconditionMG
     from benchmarks_amd.learned import _stream, smooth_rhs, train_coarse
     mg = PreconditionMG(p, p + 1, 1, n, deform=0.05, geometry=ON_THE_FLY)
     A = mg.operator()
-    coarse = LaplaceOperator(1, 2, 1, n // 2, deform=0.05, with_fp32=True)
+    coarse_mg = PreconditionMG(1, 2, 1, n // 2, deform=0.05)  # Q1 on the coarse box
     net = LearnedPreconditioner(seed)
-    st = train_coarse(net, coarse, n_fields=n_fields, mse_steps=mse_steps,
+    st = train_coarse(net, coarse_mg, n_fields=n_fields, mse_steps=mse_steps,
                       energy_steps=energy_steps, seed=seed)
     b = smooth_rhs(A, A.dof_coordinates(), _stream(seed + 1), 32)
     x_ref = A.initialize_dof_vector()
@@ -429,7 +428,8 @@ def bench_learned(n=64, p=4, n_fields=32, mse_steps=200, energy_steps=600, degre
                        f"flexible CG to 1e-8",
            "n_dofs": A.m(),
            "network_grid": f"Q1 on {n // 2}^3 cells ({st['coarse_dims'][0]}^3 nodes)",
-           "training": f"{n_fields} Jacobi-CG solves on the coarse grid (k_max {st['k_max']}), "
+           "training": f"{n_fields} FP64 {st['data_solver']} solves on the coarse grid "
+                       f"(k_max {st['k_max']}), "
                        f"{mse_steps} MSE steps (batch 8) + {energy_steps} energy-norm steps "
                        f"(batch 2)",
            "data_s": st["data_s"], "train_mse_s": st["mse"]["seconds"],
@@ -446,7 +446,7 @@ def bench_learned(n=64, p=4, n_fields=32, mse_steps=200, energy_steps=600, degre
     out["learned_speedup_vs_jacobi"] = out["jacobi_cg"]["seconds"] / out["learned_fcg"]["seconds"]
     out["learned_speedup_vs_chebyshev"] = (out["chebyshev_fcg_without_network"]["seconds"]
                                           / out["learned_fcg"]["seconds"])
-    del mg, A, net, without, x, x_ref, b, coarse
+    del mg, A, net, without, x, x_ref, b, coarse_mg
     torch.cuda.empty_cache()
     return out
 

@@ -5,8 +5,9 @@ Data: residual -> error pairs (r, e = A^-1 r) on the node grid of a box
 operator.  r is the nodal interpolant of a sum of `n_modes` random Fourier
 modes sin(pi k1 x) sin(pi k2 y) sin(pi k3 z), integer wave numbers in
 1..k_max (default 8) and amplitudes uniform in [-1, 1] drawn from the SplitMix64 stream of
-`seed`, zero on the Dirichlet boundary; e solves A e = r in FP64 (multigrid
-CG when a PreconditionMG is given, else Jacobi-CG) to 1e-10.
+`seed`, zero on the Dirichlet boundary; e solves A e = r in FP64 to 1e-10,
+by the FP64 multigrid-preconditioned CG when a PreconditionMG is given (the
+measured configuration), else by Jacobi-CG.
 
 Training: the network sees x = r_c / ||r_c|| on crops r_c of the fields and
 learns t = e_c / (s ||r_c||), s = mean ||e|| / ||r|| over the fields (the
@@ -135,15 +136,18 @@ def train_energy(net, op, rs, es, steps: int, batch: int, lr: float, seed: int =
 def train_coarse(net, coarse, n_fields: int = 32, k_max=None, mse_steps: int = 200,
                  energy_steps: int = 600, mse_batch: int = 8, energy_batch: int = 2,
                  seed: int = 3):
-    """Train `net` on (r, e) pairs of the box operator `coarse` (Jacobi-CG
-    solves of `n_fields` smooth right-hand sides, wave numbers up to k_max,
-    default half the smallest cell count), mean squared error on whole
-    fields and then the energy-norm loss, and attach it as the coarse solve
+    """Train `net` on (r, e) pairs of a coarse box operator: `coarse` is a
+    PreconditionMG on that box (the pairs come from its FP64 multigrid CG;
+    the operator is coarse.operator()) or the box operator itself (Jacobi-CG).
+    `n_fields` smooth right-hand sides, wave numbers up to k_max (default
+    half the smallest cell count); mean squared error on whole fields, then
+    the energy-norm loss; the network is attached as the coarse solve
     (net.set_coarse).  Returns data and training statistics."""
-    cells = coarse.box_cells
+    op = coarse.operator() if isinstance(coarse, PreconditionMG) else coarse
+    cells = op.box_cells
     if cells is None:
         raise ValueError("the coarse operator must be a box operator")
-    dims = tuple(c * coarse.degree + 1 for c in cells)
+    dims = tuple(c * op.degree + 1 for c in cells)
     if k_max is None:
         k_max = max(1, min(cells) // 2)
     torch.cuda.synchronize()
@@ -153,8 +157,9 @@ def train_coarse(net, coarse, n_fields: int = 32, k_max=None, mse_steps: int = 2
     t_data = time.perf_counter() - t0
     st_mse = train_on_fields(net, dims, rs, es, steps=mse_steps, batch=mse_batch, crop=None,
                              lr=2e-3, seed=seed)
-    st_en = train_energy(net, coarse, rs, es, steps=energy_steps, batch=energy_batch, lr=1e-3,
+    st_en = train_energy(net, op, rs, es, steps=energy_steps, batch=energy_batch, lr=1e-3,
                          seed=seed)
-    net.set_coarse(coarse)
+    net.set_coarse(op)
     return {"coarse_dims": dims, "k_max": k_max, "n_fields": n_fields, "data_s": t_data,
+            "data_solver": "multigrid CG" if op is not coarse else "Jacobi-CG",
             "mse": st_mse, "energy": st_en}
