@@ -414,13 +414,12 @@ mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b, const double *in
     int batch = 4;
     for (int it = 0;;) {
       for (int k = 0; k < batch; ++k, ++it) {
-        cg_direction(op->st.ptr, op->r.ptr, inv_diag, op->pdir.ptr, op->ap.ptr, N, s);
+        cg_direction(op->st.ptr, op->r.ptr, inv_diag, op->pdir.ptr, op->ap.ptr, x, N, s);
         op->apply64(op->pdir.ptr, op->ap.ptr, s, op->part.ptr, done);
         cg_constrained(op->st.ptr, op->cons.ptr, op->n_cons, op->pdir.ptr, op->ap.ptr,
                        op->part.ptr + nb_apply, s);
         cg_alpha(op->st.ptr, op->part.ptr, nb_apply + nb_cons, s);
-        cg_update(op->st.ptr, x, op->r.ptr, op->pdir.ptr, op->ap.ptr, inv_diag, N,
-                  op->part.ptr, s);
+        cg_update(op->st.ptr, op->r.ptr, op->ap.ptr, inv_diag, N, op->part.ptr, s);
         cg_check(op->st.ptr, op->part.ptr, nb_vec, s);
       }
       MFX_HIP(hipGetLastError());
@@ -430,6 +429,7 @@ mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b, const double *in
       if (it > max_steps + 8) break;  // device enforces max_steps; this bounds the host loop
       batch = std::min(batch * 2, 64);
     }
+    cg_finish(op->st.ptr, x, op->pdir.ptr, N, s);
     CgState fin{};
     MFX_HIP(hipMemcpyAsync(&fin, op->st.ptr, sizeof(CgState), hipMemcpyDeviceToHost, s));
     MFX_HIP(hipStreamSynchronize(s));

@@ -7,7 +7,9 @@
 //   Step k: v = P r; gamma = r.v; beta = gamma / gamma_old (k > 1);
 //   p = v + beta p; v = A p; alpha = gamma / p.v; x += alpha p; r -= alpha v.
 // The r.v of step k+1 is computed in the update pass of step k, so each
-// step needs two global reductions (p.Ap, and r.r with r.Pr).
+// step needs two global reductions (p.Ap, and r.r with r.Pr).  The x update
+// of step k runs in the direction pass of step k+1, which reads p anyway
+// (one pass over N fewer per step), and in cg_finish after the last step.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,13 +38,17 @@ uint32_t cg_vec_blocks(uint64_t n);
 uint32_t cg_cons_blocks(uint64_t nc);
 void cg_init(CgState *st, const double *b, double *r, const double *invd, uint64_t n,
              double *part, double abs_tol, double rel_tol, int max_steps, hipStream_t s);
+// x += alpha p of the previous step (not on the first), p = P r + beta p, Ap = 0
 void cg_direction(const CgState *st, const double *r, const double *invd, double *p, double *ap,
-                  uint64_t n, hipStream_t s);
+                  double *x, uint64_t n, hipStream_t s);
 void cg_constrained(const CgState *st, const uint32_t *list, uint64_t nc, const double *p,
                     double *ap, double *part, hipStream_t s);
 void cg_alpha(CgState *st, const double *part, uint32_t n, hipStream_t s);
-void cg_update(const CgState *st, double *x, double *r, const double *p, const double *ap,
-               const double *invd, uint64_t n, double *part, hipStream_t s);
+// r -= alpha Ap with the partials of r.r and r.Pr
+void cg_update(const CgState *st, double *r, const double *ap, const double *invd, uint64_t n,
+               double *part, hipStream_t s);
 void cg_check(CgState *st, const double *part, uint32_t nb, hipStream_t s);
+// the last step's x += alpha p (nothing if no step was taken)
+void cg_finish(const CgState *st, double *x, const double *p, uint64_t n, hipStream_t s);
 
 }  // namespace mfx

@@ -173,19 +173,37 @@ __global__ void k_cg_start(CgState *st, const double *__restrict__ part, uint32_
   }
 }
 
-// p = P r + beta p (p = P r on the first step); Ap = 0.
+// x += alpha p (the previous step's update, which needs the old p this
+// kernel reads anyway), then p = P r + beta p (p = P r on the first step);
+// Ap = 0.
 __global__ void k_cg_direction(const CgState *st, const double *__restrict__ r,
                                const double *__restrict__ invd, double *__restrict__ p,
-                               double *__restrict__ ap, uint64_t n) {
+                               double *__restrict__ ap, double *__restrict__ x, uint64_t n) {
   if (st->done) return;
-  const double beta = st->beta;
+  const double beta = st->beta, alpha = st->alpha;
   const bool first = st->first != 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const double v = invd ? invd[i] * r[i] : r[i];
-    p[i] = first ? v : v + beta * p[i];
+    if (first) {
+      p[i] = v;
+    } else {
+      const double pi = p[i];
+      x[i] += alpha * pi;
+      p[i] = v + beta * pi;
+    }
     ap[i] = 0.0;
   }
+}
+
+// The last step's x += alpha p, after the iteration has stopped.
+__global__ void k_cg_finish(const CgState *st, double *__restrict__ x,
+                            const double *__restrict__ p, uint64_t n) {
+  if (st->steps == 0) return;
+  const double alpha = st->alpha;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    x[i] += alpha * p[i];
 }
 
 // Identity rows: Ap_c = p_c, partial sum of p_c^2 (the constrained part of p.Ap).
@@ -215,17 +233,16 @@ __global__ void k_cg_alpha(CgState *st, const double *__restrict__ part, uint32_
   }
 }
 
-// x += alpha p; r -= alpha Ap; partials of r.r and r.(P r).
-__global__ void k_cg_update(const CgState *st, double *__restrict__ x, double *__restrict__ r,
-                            const double *__restrict__ p, const double *__restrict__ ap,
-                            const double *__restrict__ invd, uint64_t n,
-                            double *__restrict__ part) {
+// r -= alpha Ap; partials of r.r and r.(P r).  (x += alpha p is applied by
+// the next direction update, or by k_cg_finish after the last step.)
+__global__ void k_cg_update(const CgState *st, double *__restrict__ r,
+                            const double *__restrict__ ap, const double *__restrict__ invd,
+                            uint64_t n, double *__restrict__ part) {
   if (st->done) return;
   const double alpha = st->alpha;
   double rr = 0.0, rz = 0.0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
     const double ri = r[i] - alpha * ap[i];
     r[i] = ri;
     rr += ri * ri;
@@ -338,8 +355,13 @@ void cg_init(CgState *st, const double *b, double *r, const double *invd, uint64
   MFX_HIP(hipGetLastError());
 }
 void cg_direction(const CgState *st, const double *r, const double *invd, double *p, double *ap,
-                  uint64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_direction, dim3(vec_grid(n)), dim3(kBlock), 0, s, st, r, invd, p, ap, n);
+                  double *x, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_direction, dim3(vec_grid(n)), dim3(kBlock), 0, s, st, r, invd, p, ap, x,
+                     n);
+}
+void cg_finish(const CgState *st, double *x, const double *p, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_finish, dim3(vec_grid(n)), dim3(kBlock), 0, s, st, x, p, n);
+  MFX_HIP(hipGetLastError());
 }
 void cg_constrained(const CgState *st, const uint32_t *list, uint64_t nc, const double *p,
                     double *ap, double *part, hipStream_t s) {
@@ -350,10 +372,9 @@ void cg_constrained(const CgState *st, const uint32_t *list, uint64_t nc, const 
 void cg_alpha(CgState *st, const double *part, uint32_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, st, part, n);
 }
-void cg_update(const CgState *st, double *x, double *r, const double *p, const double *ap,
-               const double *invd, uint64_t n, double *part, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_update, dim3(vec_grid(n)), dim3(kBlock), 0, s, st, x, r, p, ap, invd,
-                     n, part);
+void cg_update(const CgState *st, double *r, const double *ap, const double *invd, uint64_t n,
+               double *part, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_update, dim3(vec_grid(n)), dim3(kBlock), 0, s, st, r, ap, invd, n, part);
 }
 void cg_check(CgState *st, const double *part, uint32_t nb, hipStream_t s) {
   hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kBlock), 0, s, st, part, nb);
