@@ -98,13 +98,12 @@ struct mfx_batch {
     launch_copy_listed<double>(dst, src, cons.ptr, n_cons, s);
   }
   void cg_step(double *x, const double *dinv, hipStream_t s) {
-    launch_b_direction(segs.ptr, n_segs, st.ptr, r.ptr, dinv, p.ptr, ap.ptr, s);
+    launch_b_direction(segs.ptr, n_segs, st.ptr, r.ptr, dinv, p.ptr, ap.ptr, x, s);
     apply(p.ptr, ap.ptr, flags.ptr, s);
     launch_copy_listed<double>(ap.ptr, p.ptr, cons.ptr, n_cons, s);
     launch_b_pap(segs.ptr, n_segs, st.ptr, p.ptr, ap.ptr, part.ptr, s);
     launch_b_alpha(st.ptr, inst_seg.ptr, (uint32_t)ops.size(), part.ptr, s);
-    launch_b_update(segs.ptr, n_segs, st.ptr, x, r.ptr, p.ptr, ap.ptr, dinv, part.ptr, part2.ptr,
-                    s);
+    launch_b_update(segs.ptr, n_segs, st.ptr, r.ptr, ap.ptr, dinv, part.ptr, part2.ptr, s);
     launch_b_check(st.ptr, inst_seg.ptr, (uint32_t)ops.size(), part.ptr, part2.ptr, flags.ptr,
                    n_active.ptr, s);
   }
@@ -469,6 +468,7 @@ mfx_status mfx_batch_cg_solve(mfx_batch *b, double *x, const double *rhs, const 
       MFX_HIP(hipMemcpyAsync(b->h_active, b->n_active.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
       MFX_HIP(hipStreamSynchronize(s));
     }
+    launch_b_finish(b->segs.ptr, b->n_segs, b->st.ptr, x, b->p.ptr, s);
     std::vector<BatchCg> fin(n);
     MFX_HIP(hipMemcpyAsync(fin.data(), b->st.ptr, n * sizeof(BatchCg), hipMemcpyDeviceToHost, s));
     MFX_HIP(hipStreamSynchronize(s));

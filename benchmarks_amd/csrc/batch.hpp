@@ -29,19 +29,23 @@ void launch_b_residual(const Seg *segs, uint32_t n_segs, double *r, const double
 void launch_b_init(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part_rr,
                    const double *part_rz, double abs_tol, double rel_tol, int max_steps,
                    int *flags, int *n_active, hipStream_t s);
-// p = dinv r + beta p (beta = 0 on the first step), ap = 0
+// x += alpha p of the previous step (not on the first), p = dinv r + beta p
+// (beta = 0 on the first step), ap = 0
 void launch_b_direction(const Seg *segs, uint32_t n_segs, const BatchCg *st, const double *r,
-                        const double *dinv, double *p, double *ap, hipStream_t s);
+                        const double *dinv, double *p, double *ap, double *x, hipStream_t s);
+// after the iteration: the last step's x += alpha p of every instance that took a step
+void launch_b_finish(const Seg *segs, uint32_t n_segs, const BatchCg *st, double *x,
+                     const double *p, hipStream_t s);
 // partial sums of p.ap per segment
 void launch_b_pap(const Seg *segs, uint32_t n_segs, const BatchCg *st, const double *p,
                   const double *ap, double *part, hipStream_t s);
 // alpha = gamma / p.ap per instance
 void launch_b_alpha(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part,
                     hipStream_t s);
-// x += alpha p, r -= alpha ap; partial sums of r.r and r.(dinv r)
-void launch_b_update(const Seg *segs, uint32_t n_segs, const BatchCg *st, double *x, double *r,
-                     const double *p, const double *ap, const double *dinv, double *part_rr,
-                     double *part_rz, hipStream_t s);
+// r -= alpha ap; partial sums of r.r and r.(dinv r)
+void launch_b_update(const Seg *segs, uint32_t n_segs, const BatchCg *st, double *r,
+                     const double *ap, const double *dinv, double *part_rr, double *part_rz,
+                     hipStream_t s);
 // convergence test, beta, gamma; flags and *n_active as in launch_b_init
 void launch_b_check(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part_rr,
                     const double *part_rz, int *flags, int *n_active, hipStream_t s);

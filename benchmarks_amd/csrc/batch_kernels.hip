@@ -111,18 +111,34 @@ k_b_count(const int *__restrict__ flags, uint32_t n_inst, int *__restrict__ n_ac
   if (threadIdx.x == 0) *n_active = (int)a;
 }
 
+// x += alpha p of the previous step (it needs the p read here anyway), then
+// p = dinv r + beta p
 __global__ void __launch_bounds__(kBlock)
 k_b_direction(const Seg *__restrict__ segs, const BatchCg *__restrict__ st,
               const double *__restrict__ r, const double *__restrict__ dinv,
-              double *__restrict__ p, double *__restrict__ ap) {
+              double *__restrict__ p, double *__restrict__ ap, double *__restrict__ x) {
   const Seg sg = segs[blockIdx.x];
   const BatchCg &c = st[sg.inst];
   if (c.done) return;
-  const double beta = c.first ? 0.0 : c.beta;
+  const bool first = c.first != 0;
+  const double beta = first ? 0.0 : c.beta, alpha = c.alpha;
   for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) {
-    p[i] = dinv[i] * r[i] + (beta != 0.0 ? beta * p[i] : 0.0);
+    const double pi = p[i];
+    if (!first) x[i] += alpha * pi;
+    p[i] = dinv[i] * r[i] + (beta != 0.0 ? beta * pi : 0.0);
     ap[i] = 0.0;
   }
+}
+
+// the last step's x += alpha p of every instance that took a step
+__global__ void __launch_bounds__(kBlock)
+k_b_finish(const Seg *__restrict__ segs, const BatchCg *__restrict__ st, double *__restrict__ x,
+           const double *__restrict__ p) {
+  const Seg sg = segs[blockIdx.x];
+  const BatchCg &c = st[sg.inst];
+  if (c.steps == 0) return;
+  const double alpha = c.alpha;
+  for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) x[i] += alpha * p[i];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -154,18 +170,17 @@ k_b_alpha(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg,
   }
 }
 
+// r -= alpha ap (x += alpha p follows in the next direction pass or k_b_finish)
 __global__ void __launch_bounds__(kBlock)
-k_b_update(const Seg *__restrict__ segs, const BatchCg *__restrict__ st, double *__restrict__ x,
-           double *__restrict__ r, const double *__restrict__ p, const double *__restrict__ ap,
-           const double *__restrict__ dinv, double *__restrict__ part_rr,
-           double *__restrict__ part_rz) {
+k_b_update(const Seg *__restrict__ segs, const BatchCg *__restrict__ st, double *__restrict__ r,
+           const double *__restrict__ ap, const double *__restrict__ dinv,
+           double *__restrict__ part_rr, double *__restrict__ part_rz) {
   const Seg sg = segs[blockIdx.x];
   const BatchCg &c = st[sg.inst];
   if (c.done) return;
   const double alpha = c.alpha;
   double rr = 0.0, rz = 0.0;
   for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) {
-    x[i] += alpha * p[i];
     const double v = r[i] - alpha * ap[i];
     r[i] = v;
     rr += v * v;
@@ -246,9 +261,17 @@ void launch_b_init(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const
 }
 
 void launch_b_direction(const Seg *segs, uint32_t n_segs, const BatchCg *st, const double *r,
-                        const double *dinv, double *p, double *ap, hipStream_t s) {
+                        const double *dinv, double *p, double *ap, double *x, hipStream_t s) {
   if (!n_segs) return;
-  hipLaunchKernelGGL(k_b_direction, dim3(n_segs), dim3(kBlock), 0, s, segs, st, r, dinv, p, ap);
+  hipLaunchKernelGGL(k_b_direction, dim3(n_segs), dim3(kBlock), 0, s, segs, st, r, dinv, p, ap,
+                     x);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_b_finish(const Seg *segs, uint32_t n_segs, const BatchCg *st, double *x,
+                     const double *p, hipStream_t s) {
+  if (!n_segs) return;
+  hipLaunchKernelGGL(k_b_finish, dim3(n_segs), dim3(kBlock), 0, s, segs, st, x, p);
   MFX_HIP(hipGetLastError());
 }
 
@@ -265,12 +288,12 @@ void launch_b_alpha(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, cons
   MFX_HIP(hipGetLastError());
 }
 
-void launch_b_update(const Seg *segs, uint32_t n_segs, const BatchCg *st, double *x, double *r,
-                     const double *p, const double *ap, const double *dinv, double *part_rr,
-                     double *part_rz, hipStream_t s) {
+void launch_b_update(const Seg *segs, uint32_t n_segs, const BatchCg *st, double *r,
+                     const double *ap, const double *dinv, double *part_rr, double *part_rz,
+                     hipStream_t s) {
   if (!n_segs) return;
-  hipLaunchKernelGGL(k_b_update, dim3(n_segs), dim3(kBlock), 0, s, segs, st, x, r, p, ap, dinv,
-                     part_rr, part_rz);
+  hipLaunchKernelGGL(k_b_update, dim3(n_segs), dim3(kBlock), 0, s, segs, st, r, ap, dinv, part_rr,
+                     part_rz);
   MFX_HIP(hipGetLastError());
 }
 
