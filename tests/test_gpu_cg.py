@@ -137,3 +137,27 @@ def test_cg_zero_rhs_converges_at_step_zero(gpu):
     x, ctl = _solve(op, b)
     assert ctl.last_step() == 0
     assert float(x.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("steps", [1, 2, 7])
+def test_iterate_after_a_step_limit_matches_oracle(gpu, steps):
+    """x after `steps` Jacobi-CG steps that stop at the step limit equals the
+    oracle's iterate after the same steps: the x update of each step is
+    applied in the next step's direction pass, and the last one after the
+    loop, so a solve that stops early must still hold every update."""
+    from benchmarks_amd.laplace import (DiagonalMatrix, LaplaceOperator, NoConvergence,
+                                        ReductionControl, SolverCG)
+    n = (4, 3, 5)
+    op = LaplaceOperator(3, 4, O.GLL, n_cells=n, deform=0.05)
+    P = O.make_problem(3, 4, O.GLL, n, deform=0.05)
+    b = O.splitmix64_uniform(11, op.m())
+    b[P.constrained.astype(bool)] = 0.0
+    x = op.initialize_dof_vector()
+    with pytest.raises(NoConvergence):
+        SolverCG(ReductionControl(steps, 1e-300, 1e-14)).solve(
+            op, x, torch.as_tensor(b, device="cuda"), DiagonalMatrix(op.compute_inverse_diagonal()))
+    xo, r, _ = P.cg(b, inv_diag=P.inverse_diagonal(), max_steps=steps, abs_tol=1e-300,
+                    rel_tol=1e-14)
+    assert r.steps == steps
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - xo) <= 1e-12 * np.linalg.norm(xo)
