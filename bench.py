@@ -401,10 +401,12 @@ def bench_learned(n=64, p=4, n_fields=32, mse_steps=200, energy_steps=600, degre
     Chebyshev(Jacobi) steps (FP32), the U-Net as the coarse solve on the
     node grid of Q1 on (n/2)^3 cells (restricted residual in, prolongated
     correction out, energy-optimal weight), `degree` Chebyshev steps again.
-    The network is trained on (residual, error) pairs of that Q1 operator
-    (FP64 multigrid-CG solves of `n_fields` smooth right-hand sides, 32
-    random Fourier modes, wave numbers up to n/4, SplitMix64 seed 3): mean
-    squared error, then the energy-norm loss.  Flexible CG to 1e-8 on a fresh smooth
+    The network is trained on (residual, error) pairs of that Q1 operator,
+    solved by its FP64 multigrid CG: half are smooth coarse fields (32
+    random Fourier modes, wave numbers up to n/4, SplitMix64 seed 3), half
+    restricted residuals of smooth fine fields (wave numbers up to n/2)
+    after the cycle's Chebyshev pre-smoothing.  Mean squared error, then the
+    energy-norm loss.  Flexible CG to 1e-8 on a fresh smooth
     right-hand side, next to the same cycle without the network, Jacobi-CG
     and multigrid CG; errors against an FP64 solution certified to 1e-12."""
     import torch
@@ -416,7 +418,7 @@ def bench_learned(n=64, p=4, n_fields=32, mse_steps=200, energy_steps=600, degre
     coarse_mg = PreconditionMG(1, 2, 1, n // 2, deform=0.05)  # Q1 on the coarse box
     net = LearnedPreconditioner(seed)
     st = train_coarse(net, coarse_mg, n_fields=n_fields, mse_steps=mse_steps,
-                      energy_steps=energy_steps, seed=seed)
+                      energy_steps=energy_steps, seed=seed, fine=A, smoother_degree=degree)
     b = smooth_rhs(A, A.dof_coordinates(), _stream(seed + 1), 32)
     x_ref = A.initialize_dof_vector()
     _timed_solve(A, x_ref, b, mg, 1e-12, 200)
@@ -429,7 +431,9 @@ def bench_learned(n=64, p=4, n_fields=32, mse_steps=200, energy_steps=600, degre
            "n_dofs": A.m(),
            "network_grid": f"Q1 on {n // 2}^3 cells ({st['coarse_dims'][0]}^3 nodes)",
            "training": f"{n_fields} FP64 {st['data_solver']} solves on the coarse grid "
-                       f"(k_max {st['k_max']}), "
+                       f"({n_fields - st['n_restricted_fields']} coarse fields up to k "
+                       f"{st['k_max']}, {st['n_restricted_fields']} restricted pre-smoothed "
+                       f"fine fields up to k {st['fine_k_max']}), "
                        f"{mse_steps} MSE steps (batch 8) + {energy_steps} energy-norm steps "
                        f"(batch 2)",
            "data_s": st["data_s"], "train_mse_s": st["mse"]["seconds"],

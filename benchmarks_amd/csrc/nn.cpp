@@ -101,7 +101,8 @@ struct mfx_nn {
   uint64_t sm_n = 0;
   double lmax = 0.0, lmin = 0.0;
   DBuf<float> dinv32, r32, z32, t32, c32, n32, an32;
-  DBuf<double> tmp64, tmp64b, tmp64c;
+  DBuf<double> tmp64, tmp64b, tmp64c;  // sized for sm_op by setup_smoother only
+  DBuf<double> en64, en64b, en64c;     // energy-loss training, sized for its operator
   double last_omega = 0.0;
   // coarse correction: the network on another box operator's node grid
   mfx_op *coarse_op = nullptr;
@@ -111,7 +112,7 @@ struct mfx_nn {
   DBuf<double> rc64;
 
   void setup_smoother(mfx_op &op, hipStream_t s) {
-    if (sm_op == &op && sm_n == op.N) return;
+    if (sm_op == &op && sm_n == op.N && tmp64.n == op.N && tmp64c.n == op.N) return;
     MFX_REQUIRE(op.D32.ptr, "the smoothed preconditioner needs an operator made with with_fp32");
     const uint64_t N = op.N;
     tmp64.alloc(N);
@@ -615,10 +616,11 @@ mfx_status mfx_nn_train_energy(mfx_nn *nn, mfx_op *op, int batch, const double *
     uint32_t d[3];
     for (int k = 0; k < 3; ++k) d[k] = op->box.n[k] * op->p + 1;
     nn->ensure(d, true);
-    if (nn->tmp64.n != N) {
-      nn->tmp64.alloc(N);
-      nn->tmp64b.alloc(N);
-      nn->tmp64c.alloc(N);
+    // its own scratch: the smoother's vectors stay sized for the smoothed operator
+    if (nn->en64.n != N) {
+      nn->en64.alloc(N);
+      nn->en64b.alloc(N);
+      nn->en64c.alloc(N);
     }
     launch_fill_f32(nn->grad.ptr, 0.0f, n_params_total(), s);
     double total = 0.0;
@@ -629,13 +631,13 @@ mfx_status mfx_nn_train_energy(mfx_nn *nn, mfx_op *op, int batch, const double *
       MFX_REQUIRE(rn > 0.0 && eae > 0.0, "training pair with zero residual or energy");
       nn_scale_to_f32(rb, nn->x32.ptr, 1.0 / rn, N, s);
       nn->forward(s);
-      nn_scale_to_f64(nn->y32.ptr, nn->tmp64.ptr, rn * nn->out_scale, N, s);
-      launch_set_listed(nn->tmp64.ptr, 0.0, op->cons.ptr, op->n_cons, s);
-      launch_residual_f64(nn->tmp64b.ptr, nn->tmp64.ptr, eb, N, s);  // pred - e
-      vmult64(*op, nn->tmp64c.ptr, nn->tmp64b.ptr, s);
-      total += dot64(nn->tmp64b.ptr, nn->tmp64c.ptr, N, nn->dpart, s) / eae;
-      launch_set_listed(nn->tmp64c.ptr, 0.0, op->cons.ptr, op->n_cons, s);
-      nn_scale_to_f32(nn->tmp64c.ptr, nn->dy.ptr, 2.0 * rn * nn->out_scale / (eae * batch), N, s);
+      nn_scale_to_f64(nn->y32.ptr, nn->en64.ptr, rn * nn->out_scale, N, s);
+      launch_set_listed(nn->en64.ptr, 0.0, op->cons.ptr, op->n_cons, s);
+      launch_residual_f64(nn->en64b.ptr, nn->en64.ptr, eb, N, s);  // pred - e
+      vmult64(*op, nn->en64c.ptr, nn->en64b.ptr, s);
+      total += dot64(nn->en64b.ptr, nn->en64c.ptr, N, nn->dpart, s) / eae;
+      launch_set_listed(nn->en64c.ptr, 0.0, op->cons.ptr, op->n_cons, s);
+      nn_scale_to_f32(nn->en64c.ptr, nn->dy.ptr, 2.0 * rn * nn->out_scale / (eae * batch), N, s);
       nn->backward(s);
     }
     nn->step += 1;

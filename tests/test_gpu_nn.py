@@ -218,8 +218,9 @@ def test_coarse_correction_matches_composition(gpu):
 def test_trained_coarse_network_beats_jacobi_in_work(gpu):
     """The measured configuration at small size: Q4 on 16^3 cells, the
     network as the coarse solve on the Q1 grid of 8^3 cells, trained on
-    (residual, error) pairs of that Q1 operator from its FP64 multigrid CG
-    (train_coarse).  What the
+    (residual, error) pairs of that Q1 operator from its FP64 multigrid CG,
+    half of them restricted pre-smoothed fine residuals (train_coarse).
+    What the
     trained network adds: flexible-CG steps against the same cycle without
     it, and fine-operator applications (2 degree + 2 per FCG step, 1 per
     Jacobi-CG step) against Jacobi-CG."""
@@ -229,8 +230,9 @@ def test_trained_coarse_network_beats_jacobi_in_work(gpu):
     A = LaplaceOperator(4, 5, O.GLL, 16, deform=0.05, with_fp32=True, geometry=1)
     Cmg = PreconditionMG(1, 2, O.GLL, 8, deform=0.05)
     net = LearnedPreconditioner(3)
-    st = train_coarse(net, Cmg, n_fields=32, mse_steps=100, energy_steps=300, seed=3)
-    assert st["data_solver"] == "multigrid CG"
+    st = train_coarse(net, Cmg, n_fields=32, mse_steps=100, energy_steps=300, seed=3, fine=A,
+                      smoother_degree=4)
+    assert st["data_solver"] == "multigrid CG" and st["n_restricted_fields"] == 16
     assert st["energy"]["loss_last"] < 0.5 * st["energy"]["loss_first"], st["energy"]
     deg = 4
     net.set_mode(deg, True)
