@@ -4,7 +4,8 @@
 //
 // Vector kernels run one workgroup per segment (a contiguous range inside
 // one instance), skip segments of converged instances, and write one partial
-// per segment.  The scalar kernels run one workgroup per instance, which sums
+// per segment.  Their segment loops are unrolled by 4 so that each thread has
+// several independent loads in flight.  The scalar kernels run one workgroup per instance, which sums
 // that instance's segment partials in a fixed order, so every reduction is
 // deterministic.
 #include <algorithm>
@@ -48,6 +49,7 @@ k_b_residual(const Seg *__restrict__ segs, double *__restrict__ r, const double 
              double *__restrict__ part_rr, double *__restrict__ part_rz) {
   const Seg sg = segs[blockIdx.x];
   double rr = 0.0, rz = 0.0;
+#pragma unroll 4
   for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) {
     const double v = b[i] - t[i];
     r[i] = v;
@@ -122,6 +124,7 @@ k_b_direction(const Seg *__restrict__ segs, const BatchCg *__restrict__ st,
   if (c.done) return;
   const bool first = c.first != 0;
   const double beta = first ? 0.0 : c.beta, alpha = c.alpha;
+#pragma unroll 4
   for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) {
     const double pi = p[i];
     if (!first) x[i] += alpha * pi;
@@ -138,6 +141,7 @@ k_b_finish(const Seg *__restrict__ segs, const BatchCg *__restrict__ st, double 
   const BatchCg &c = st[sg.inst];
   if (c.steps == 0) return;
   const double alpha = c.alpha;
+#pragma unroll 4
   for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) x[i] += alpha * p[i];
 }
 
@@ -147,6 +151,7 @@ k_b_pap(const Seg *__restrict__ segs, const BatchCg *__restrict__ st,
   const Seg sg = segs[blockIdx.x];
   if (st[sg.inst].done) return;
   double s = 0.0, unused = 0.0;
+#pragma unroll 4
   for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) s += p[i] * ap[i];
   block_sum2(s, unused);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -180,6 +185,7 @@ k_b_update(const Seg *__restrict__ segs, const BatchCg *__restrict__ st, double 
   if (c.done) return;
   const double alpha = c.alpha;
   double rr = 0.0, rz = 0.0;
+#pragma unroll 4
   for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) {
     const double v = r[i] - alpha * ap[i];
     r[i] = v;
