@@ -117,16 +117,16 @@ void autotune(mfx_batch &b, hipStream_t s) {
   dst.alloc(b.N);
   launch_random(src.ptr, b.N, 7, s);
   launch_fill(dst.ptr, 0.0, b.N, s);
-  hipEvent_t e0, e1;
-  MFX_HIP(hipEventCreate(&e0));
-  MFX_HIP(hipEventCreate(&e1));
-  struct Ev {
-    hipEvent_t a, b;
+  struct Ev {  // destroys what was created, also when the second create throws
+    hipEvent_t a = nullptr, b = nullptr;
     ~Ev() {
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
     }
-  } ev{e0, e1};
+  } ev;
+  MFX_HIP(hipEventCreate(&ev.a));
+  MFX_HIP(hipEventCreate(&ev.b));
+  const hipEvent_t e0 = ev.a, e1 = ev.b;
   for (auto &g : b.groups) {
     const int nv = g.n_variants();
     g.ms.assign(nv, 0.0);
