@@ -107,6 +107,7 @@ _SIGS = {
     "mfx_cg_solve": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double,
                                C.POINTER(CgResult), _P]),
     "mfx_vec_random": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P]),
+    "mfx_vec_sine_modes": (C.c_int, [_P, C.c_uint64, C.c_int, _P, _P, _P, _P]),
     "mfx_vec_zero_constrained": (C.c_int, [_P, _P, _P]),
     "mfx_vec_dot": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_double), _P]),
     "mfx_mg_create": (C.c_int, [C.POINTER(MgDesc), C.POINTER(C.c_void_p)]),
@@ -153,6 +154,9 @@ _SIGS = {
     "mfx_nn_vmult": (C.c_int, [_P, _P, _P, _P, _P]),
     "mfx_nn_fcg_solve": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double,
                                    C.POINTER(CgResult), _P]),
+    "mfx_nn_restricted_residual": (C.c_int, [_P, _P, _P, _P, _P]),
+    "mfx_nn_set_fallback": (C.c_int, [_P, _P, C.c_int, C.c_double]),
+    "mfx_nn_fallback_step": (C.c_int, [_P]),
 }
 
 _lib = None

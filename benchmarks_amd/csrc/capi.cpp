@@ -454,6 +454,23 @@ mfx_status mfx_vec_random(double *x, uint64_t n, uint64_t seed, void *stream) {
   });
 }
 
+mfx_status mfx_vec_sine_modes(const double *coords, uint64_t n, int n_modes,
+                              const int32_t *wave_numbers, const double *amplitudes, double *out,
+                              void *stream) {
+  return guard([&] {
+    MFX_REQUIRE((coords && out) || n == 0, "null pointer argument");
+    MFX_REQUIRE(n_modes >= 0 && n_modes <= SineModes::kMax, "n_modes must be in 0..64");
+    MFX_REQUIRE(n_modes == 0 || (wave_numbers && amplitudes), "null mode arrays");
+    SineModes m{};
+    m.n = n_modes;
+    for (int i = 0; i < n_modes; ++i) {
+      for (int d = 0; d < 3; ++d) m.k[3 * i + d] = wave_numbers[3 * i + d];
+      m.a[i] = amplitudes[i];
+    }
+    launch_sine_modes(coords, n, m, out, as_stream(stream));
+  });
+}
+
 mfx_status mfx_vec_zero_constrained(mfx_op *op, double *x, void *stream) {
   return guard([&] {
     MFX_REQUIRE(op && x, "null pointer argument");

@@ -93,6 +93,15 @@ void launch_scatter_add(const uint32_t *map, uint64_t n, const double *evec, dou
                         hipStream_t s);
 void launch_invert_diagonal(double *d, uint64_t n, hipStream_t s);
 void launch_random(double *x, uint64_t n, uint64_t seed, hipStream_t s);
+// sum of up to kMax sine modes at n points (mfx_vec_sine_modes), passed by value
+struct SineModes {
+  static constexpr int kMax = 64;
+  int n;
+  int k[3 * kMax];
+  double a[kMax];
+};
+void launch_sine_modes(const double *xyz, uint64_t n, const SineModes &m, double *out,
+                       hipStream_t s);
 // deterministic two-stage dot: partials[nb] then final sum into *result
 uint32_t dot_n_blocks(uint64_t n);
 void launch_dot(const double *x, const double *y, uint64_t n, double *partials,

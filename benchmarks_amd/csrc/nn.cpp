@@ -97,22 +97,29 @@ struct mfx_nn {
   // two-level mode: Chebyshev smoothing around the network
   int smooth_degree = 0;
   bool use_net = true;
-  const mfx_op *sm_op = nullptr;
+  uint64_t sm_serial = 0;  // the operator the smoother is set up for (0: none)
   uint64_t sm_n = 0;
   double lmax = 0.0, lmin = 0.0;
   DBuf<float> dinv32, r32, z32, t32, c32, n32, an32;
-  DBuf<double> tmp64, tmp64b, tmp64c;  // sized for sm_op by setup_smoother only
+  DBuf<double> tmp64, tmp64b, tmp64c;  // sized by setup_smoother only, for its operator
   DBuf<double> en64, en64b, en64c;     // energy-loss training, sized for its operator
   double last_omega = 0.0;
   // coarse correction: the network on another box operator's node grid
   mfx_op *coarse_op = nullptr;
-  const mfx_op *tr_fine = nullptr;
+  // the operator pair the transfer and its scratch were built for (0: none)
+  uint64_t tr_fine_serial = 0, tr_coarse_serial = 0;
   Transfer tr;
   DBuf<float> rc32, ec32, ta32, tb32;
   DBuf<double> rc64;
+  // fallback of the flexible CG: one V-cycle of this multigrid per step once
+  // the learned cycle stagnates (the caller keeps it alive)
+  mfx_mg *fb_mg = nullptr;
+  int fb_window = 8;
+  double fb_min_reduction = 2.0;
+  int fb_step = -1;  // step at which the last solve switched to it (-1: it did not)
 
   void setup_smoother(mfx_op &op, hipStream_t s) {
-    if (sm_op == &op && sm_n == op.N && tmp64.n == op.N && tmp64c.n == op.N) return;
+    if (sm_serial == op.serial && sm_n == op.N && tmp64.n == op.N && tmp64c.n == op.N) return;
     MFX_REQUIRE(op.D32.ptr, "the smoothed preconditioner needs an operator made with with_fp32");
     const uint64_t N = op.N;
     tmp64.alloc(N);
@@ -130,7 +137,7 @@ struct mfx_nn {
     an32.alloc(N);
     tmp64b.alloc(N);
     tmp64c.alloc(N);
-    sm_op = &op;
+    sm_serial = op.serial;
     sm_n = N;
   }
 
@@ -262,21 +269,50 @@ struct mfx_nn {
 
   // z32 += omega P c with c = ||rho_c|| s net(rho_c / ||rho_c||) on the
   // coarse node grid, rho_c = R rho (rho in t32), omega energy-optimal
-  void coarse_correction(mfx_op &op, hipStream_t s) {
+  // rc32 = R t32 with the coarse constrained rows zeroed, rc64 its FP64
+  // copy: the residual the network sees on the coarse grid (transfer and
+  // scratch built for the operator pair on first use)
+  void restrict_residual(mfx_op &op, hipStream_t s) {
     mfx_op &C = *coarse_op;
     const uint64_t N = op.N, Nc = C.N;
-    if (tr_fine != &op) {
+    if (tr_fine_serial != op.serial || tr_coarse_serial != C.serial || ta32.n != N ||
+        tb32.n != N || rc32.n != Nc) {
       build_transfer(op, C, tr, true, false);
       rc32.alloc(Nc);
       ec32.alloc(Nc);
       rc64.alloc(Nc);
       ta32.alloc(N);
       tb32.alloc(N);
-      tr_fine = &op;
+      tr_fine_serial = op.serial;
+      tr_coarse_serial = C.serial;
     }
+    // the smoother's vectors (t32, n32, an32) were set up for this operator
+    MFX_REQUIRE(sm_serial == op.serial && t32.n == N && n32.n == N && an32.n == N,
+                "coarse correction: smoother scratch does not match the fine operator");
+    MFX_REQUIRE(count(tr.fine) == N && count(tr.coarse) == Nc,
+                "coarse correction: transfer does not match the operators");
     restrict_<float>(tr.R32, tr.fine, tr.coarse, t32.ptr, ta32.ptr, tb32.ptr, rc32.ptr, s);
     launch_set_listed_f32(rc32.ptr, 0.0f, C.cons.ptr, C.n_cons, s);
     launch_to_double(rc32.ptr, rc64.ptr, Nc, s);
+  }
+
+  // r32 = r (constrained rows 0), z32 = the pre-smoothing from zero, and
+  // t32 = r32 - A z32: the first half of the two-level cycle
+  void presmooth_residual(mfx_op &op, const double *r, hipStream_t s) {
+    const uint64_t N = op.N;
+    launch_to_float(r, r32.ptr, N, s);
+    launch_set_listed_f32(r32.ptr, 0.0f, op.cons.ptr, op.n_cons, s);
+    chebyshev(op, z32.ptr, r32.ptr, dinv32.ptr, t32.ptr, c32.ptr, smooth_degree, lmin, lmax, true, s);
+    vmult32(op, t32.ptr, z32.ptr, s);
+    launch_residual_f32(t32.ptr, r32.ptr, t32.ptr, N, s);
+  }
+
+  // z32 += omega P c with c = ||rho_c|| s net(rho_c / ||rho_c||) on the
+  // coarse node grid, rho_c = R rho (rho in t32), omega energy-optimal
+  void coarse_correction(mfx_op &op, hipStream_t s) {
+    mfx_op &C = *coarse_op;
+    const uint64_t N = op.N, Nc = C.N;
+    restrict_residual(op, s);
     const double rn = std::sqrt(dot64(rc64.ptr, rc64.ptr, Nc, dpart, s));
     if (!(rn > 0.0)) return;
     uint32_t d[3];
@@ -301,20 +337,21 @@ struct mfx_nn {
 
   void precondition_two_level(mfx_op &op, double *z, const double *r, hipStream_t s) {
     setup_smoother(op, s);
+    last_omega = 0.0;  // stays 0 when the network adds nothing to this application
     const uint64_t N = op.N;
-    launch_to_float(r, r32.ptr, N, s);
-    launch_set_listed_f32(r32.ptr, 0.0f, op.cons.ptr, op.n_cons, s);
-    chebyshev(op, z32.ptr, r32.ptr, dinv32.ptr, t32.ptr, c32.ptr, smooth_degree, lmin, lmax, true, s);
     if (use_net && coarse_op) {
-      vmult32(op, t32.ptr, z32.ptr, s);
-      launch_residual_f32(t32.ptr, r32.ptr, t32.ptr, N, s);
+      presmooth_residual(op, r, s);
       coarse_correction(op, s);
-    } else if (use_net) {
+    } else if (!use_net) {
+      launch_to_float(r, r32.ptr, N, s);
+      launch_set_listed_f32(r32.ptr, 0.0f, op.cons.ptr, op.n_cons, s);
+      chebyshev(op, z32.ptr, r32.ptr, dinv32.ptr, t32.ptr, c32.ptr, smooth_degree, lmin, lmax, true,
+                s);
+    } else {
+      presmooth_residual(op, r, s);
       uint32_t d[3];
       for (int k = 0; k < 3; ++k) d[k] = op.box.n[k] * op.p + 1;
       ensure(d, false);
-      vmult32(op, t32.ptr, z32.ptr, s);
-      launch_residual_f32(t32.ptr, r32.ptr, t32.ptr, N, s);
       launch_to_double(t32.ptr, tmp64.ptr, N, s);
       const double rn = std::sqrt(dot64(tmp64.ptr, tmp64.ptr, N, dpart, s));
       if (rn > 0.0) {
@@ -436,7 +473,8 @@ mfx_status mfx_nn_set_coarse(mfx_nn *nn, mfx_op *coarse) {
     MFX_REQUIRE(nn, "null pointer argument");
     MFX_REQUIRE(!coarse || coarse->is_box, "the coarse operator must come from mfx_op_create_box");
     nn->coarse_op = coarse;
-    nn->tr_fine = nullptr;
+    nn->tr_fine_serial = 0;
+    nn->tr_coarse_serial = 0;
   });
 }
 
@@ -535,17 +573,52 @@ mfx_status mfx_nn_vmult(mfx_nn *nn, mfx_op *op, double *z, const double *r, void
   });
 }
 
+mfx_status mfx_nn_restricted_residual(mfx_nn *nn, mfx_op *op, const double *r, double *rho_c,
+                                      void *stream) {
+  return guard([&] {
+    MFX_REQUIRE(nn && op && r && rho_c, "null pointer argument");
+    MFX_REQUIRE(op->is_box, "the network runs on the node grid of a box operator");
+    MFX_REQUIRE(nn->smooth_degree > 0 && nn->coarse_op,
+                "needs the two-level mode (smoother degree > 0) and a coarse operator");
+    hipStream_t s = as_stream(stream);
+    nn->setup_smoother(*op, s);
+    nn->presmooth_residual(*op, r, s);
+    nn->restrict_residual(*op, s);
+    MFX_HIP(hipMemcpyAsync(rho_c, nn->rc64.ptr, nn->coarse_op->N * sizeof(double),
+                           hipMemcpyDeviceToDevice, s));
+  });
+}
+
 // Flexible CG with up to kFcgDirs stored directions (Notay's FCG(m)): the
 // new direction is the preconditioned residual made A-orthogonal to the last
 // m directions, p = z - sum_j (z.Ap_j / p_j.Ap_j) p_j, then
 // alpha = p.r / p.Ap.  With a linear SPD preconditioner it is PCG; with the
 // network it keeps the Krylov acceleration the single-direction form loses.
 // Stopping rule and result fields of mfx_cg_solve.
+//
+// Fallback (mfx_nn_set_fallback): the learned cycle is judged stagnant when
+// the residual falls by less than fb_min_reduction over the last fb_window
+// steps, when the network's correction was rejected (omega <= 0: it does not
+// lower the energy norm of the error) in fb_window applications in a row, or
+// when a direction breaks down (p.Ap <= 0, e.g. a zero preconditioned
+// residual).  The solve then continues with one V-cycle of the fallback
+// multigrid per step; flexible CG needs no restart for the change of
+// preconditioner.  This mirrors the only failure path of the reference's
+// solver, which raises NoConvergence (bp5_kokkos/benchmark.cc:366-374),
+// with a working method in place of the exception.
 mfx_status mfx_nn_fcg_solve(mfx_nn *nn, mfx_op *op, double *x, const double *b, int max_steps,
                             double abs_tol, double rel_tol, mfx_cg_result *res, void *stream) {
   return guard([&] {
     MFX_REQUIRE(nn && op && x && b && res, "null pointer argument");
     MFX_REQUIRE(op->is_box, "the network runs on the node grid of a box operator");
+    MFX_REQUIRE(max_steps >= 0, "max_steps must be >= 0");
+    if (nn->fb_mg) {
+      mfx_mg_level_info li{};
+      const mfx_status st = mfx_mg_get_level(nn->fb_mg, 0, &li);
+      if (st != MFX_OK) throw Error{st, mfx_last_error()};
+      MFX_REQUIRE(li.op && li.op->serial == op->serial,
+                  "the fallback multigrid's fine operator must be the solve's operator");
+    }
     constexpr int kFcgDirs = 4;
     hipStream_t s = as_stream(stream);
     const uint64_t N = op->N;
@@ -565,10 +638,21 @@ mfx_status mfx_nn_fcg_solve(mfx_nn *nn, mfx_op *op, double *x, const double *b, 
     double rn = std::sqrt(dot64(r, r, N, nn->dpart, s));
     res->initial_value = rn;
     const double tol = std::max(abs_tol, rel_tol * rn);
+    std::vector<double> hist{rn};
+    nn->fb_step = -1;
+    const bool net_judged = nn->use_net && nn->smooth_degree > 0;
+    int rejected_run = 0;  // consecutive applications whose correction was rejected
     int steps = 0;
     bool converged = rn <= tol, breakdown = false;
+    auto engage = [&]() { nn->fb_step = steps; };
     while (!converged && steps < max_steps) {
-      nn->precondition(*op, z, r, s);
+      if (nn->fb_step >= 0) {
+        const mfx_status st = mfx_mg_vmult(nn->fb_mg, z, r, stream);
+        if (st != MFX_OK) throw Error{st, mfx_last_error()};
+      } else {
+        nn->precondition(*op, z, r, s);
+        rejected_run = (net_judged && !(nn->last_omega > 0.0)) ? rejected_run + 1 : 0;
+      }
       const int slot = steps % kFcgDirs, stored = std::min(steps, kFcgDirs);
       double beta[kFcgDirs] = {};
       for (int j = 0; j < stored; ++j)
@@ -579,6 +663,10 @@ mfx_status mfx_nn_fcg_solve(mfx_nn *nn, mfx_op *op, double *x, const double *b, 
       vmult64(*op, Q(slot), P(slot), s);
       pq[slot] = dot64(P(slot), Q(slot), N, nn->dpart, s);
       if (!(pq[slot] > 0.0)) {
+        if (nn->fb_mg && nn->fb_step < 0) {
+          engage();  // redo this step with the multigrid
+          continue;
+        }
         breakdown = true;
         break;
       }
@@ -586,7 +674,13 @@ mfx_status mfx_nn_fcg_solve(mfx_nn *nn, mfx_op *op, double *x, const double *b, 
       launch_pcg_update(x, r, P(slot), Q(slot), alpha, N, s);
       ++steps;
       rn = std::sqrt(dot64(r, r, N, nn->dpart, s));
+      hist.push_back(rn);
       converged = rn <= tol;
+      if (!converged && nn->fb_mg && nn->fb_step < 0) {
+        const int w = nn->fb_window;
+        const bool slow = steps >= w && rn * nn->fb_min_reduction > hist[steps - w];
+        if (slow || rejected_run >= w) engage();
+      }
     }
     MFX_HIP(hipStreamSynchronize(s));
     res->steps = steps;
@@ -599,6 +693,20 @@ mfx_status mfx_nn_fcg_solve(mfx_nn *nn, mfx_op *op, double *x, const double *b, 
                       " residual=" + std::to_string(rn)};
   });
 }
+
+mfx_status mfx_nn_set_fallback(mfx_nn *nn, mfx_mg *mg, int window, double min_reduction) {
+  return guard([&] {
+    MFX_REQUIRE(nn, "null pointer argument");
+    MFX_REQUIRE(window >= 0 && window <= 1000, "window must be in 0..1000 (0: 8)");
+    MFX_REQUIRE(min_reduction == 0.0 || min_reduction > 1.0,
+                "min_reduction must exceed 1 (0: 2)");
+    nn->fb_mg = mg;
+    nn->fb_window = window ? window : 8;
+    nn->fb_min_reduction = min_reduction > 0.0 ? min_reduction : 2.0;
+  });
+}
+
+int mfx_nn_fallback_step(const mfx_nn *nn) { return nn ? nn->fb_step : -1; }
 
 // One Adam step on the energy-norm loss of whole fields of a box operator:
 //   L = (1/B) sum_b (pred_b - e_b)^T A (pred_b - e_b) / e_b^T A e_b,

@@ -2,6 +2,7 @@
 // C-ABI translation units share (capi.cpp, mg.cpp).
 #pragma once
 
+#include <atomic>
 #include <memory>
 #include <new>
 #include <string>
@@ -78,11 +79,20 @@ struct DBuf {
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Process-wide creation count of operators.  Caches that belong to one
+// operator key on this serial, never on its address: a destroyed
+// operator's heap address can be reused by the next one created.
+inline uint64_t next_op_serial() {
+  static std::atomic<uint64_t> count{0};
+  return ++count;
+}
+
 }  // namespace mfx
 
 using namespace mfx;
 
 struct mfx_op {
+  const uint64_t serial = mfx::next_op_serial();
   // set for operators made by mfx_op_create_box: the mesh they were built on
   bool is_box = false;
   mfx::BoxGeom box{};

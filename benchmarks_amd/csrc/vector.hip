@@ -96,6 +96,21 @@ __global__ void k_random(double *x, uint64_t n, uint64_t seed) {
   }
 }
 
+// Sum of sine modes at given points (x, y, z interleaved):
+// out_i = sum_m a_m sin(pi k0_m x_i) sin(pi k1_m y_i) sin(pi k2_m z_i),
+// the modes summed in index order.
+__global__ void k_sine_modes(const double *__restrict__ xyz, uint64_t n, SineModes m,
+                             double *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    double f = 0.0;
+    for (int k = 0; k < m.n; ++k)
+      f += m.a[k] * (sinpi(m.k[3 * k] * x) * sinpi(m.k[3 * k + 1] * y) * sinpi(m.k[3 * k + 2] * z));
+    out[i] = f;
+  }
+}
+
 __global__ void k_gather(const uint32_t *__restrict__ map, uint64_t n,
                          const double *__restrict__ src, double *__restrict__ evec) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -169,7 +184,8 @@ __global__ void k_cg_start(CgState *st, const double *__restrict__ part, uint32_
     st->max_steps = max_steps;
     st->breakdown = 0;
     if (st->history && st->history_len > 0) st->history[0] = rn;
-    st->done = (rn <= st->tol) ? 1 : (isnan(rn) ? 2 : 0);
+    // deal.II's SolverControl::check fails at step 0 when max_steps <= 0
+    st->done = (rn <= st->tol) ? 1 : ((isnan(rn) || max_steps <= 0) ? 2 : 0);
   }
 }
 
@@ -331,6 +347,13 @@ void launch_scatter_add(const uint32_t *map, uint64_t n, const double *evec, dou
                         hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_scatter_add, dim3(vec_grid(n)), dim3(kBlock), 0, s, map, n, evec, dst);
+  MFX_HIP(hipGetLastError());
+}
+void launch_sine_modes(const double *xyz, uint64_t n, const SineModes &m, double *out,
+                       hipStream_t s) {
+  MFX_REQUIRE(m.n >= 0 && m.n <= SineModes::kMax, "sine modes: count out of range");
+  if (!n) return;
+  hipLaunchKernelGGL(k_sine_modes, dim3(vec_grid(n)), dim3(kBlock), 0, s, xyz, n, m, out);
   MFX_HIP(hipGetLastError());
 }
 uint32_t dot_n_blocks(uint64_t n) { return vec_grid(n); }

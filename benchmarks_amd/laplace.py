@@ -591,6 +591,29 @@ class LearnedPreconditioner:
         check(lib().mfx_nn_vmult(self._h, op._h, _ptr(z, F64, op.m()), _ptr(r, F64, op.m()),
                                  _stream()))
 
+    def restricted_residual(self, op: LaplaceOperator, r: torch.Tensor, rho_c: torch.Tensor):
+        """rho_c = R (r - A z), z the cycle's pre-smoothing of A z = r: the
+        residual the two-level cycle hands the network, on the coarse
+        operator's DoFs (needs set_mode(k > 0) and set_coarse)."""
+        n_c = self._coarse.m() if getattr(self, "_coarse", None) is not None else rho_c.numel()
+        check(lib().mfx_nn_restricted_residual(self._h, op._h, _ptr(r, F64, op.m()),
+                                               _ptr(rho_c, F64, n_c), _stream()))
+
+    def set_fallback(self, mg, window: int = 8, min_reduction: float = 2.0) -> None:
+        """The multigrid (a PreconditionMG whose operator() is the solve's
+        operator) flexible CG switches to when the learned cycle stagnates:
+        the residual falls by less than min_reduction over `window` steps,
+        the network's correction is rejected `window` times in a row, or a
+        direction breaks down.  None: no fallback."""
+        self._fallback = mg  # keep the multigrid alive while in use
+        check(lib().mfx_nn_set_fallback(self._h, None if mg is None else mg._h, window,
+                                        min_reduction))
+
+    @property
+    def fallback_step(self) -> int:
+        """Step at which the last solve switched to the fallback (-1: it did not)."""
+        return int(lib().mfx_nn_fallback_step(self._h))
+
 
 def batch_instances(count: int = 64, seed: int = 2, n_min: int = 4, n_max: int = 32,
                     degrees=(2, 3, 4, 5, 6)):
@@ -660,6 +683,20 @@ def scatter_add(map_, n_cells, nodes_per_cell, evec, dst):
                                     nodes_per_cell,
                                     _ptr(evec, F64, n_cells * nodes_per_cell), _ptr(dst, F64),
                                     _stream()))
+
+
+def sine_modes(coords: torch.Tensor, wave_numbers, amplitudes, out: torch.Tensor) -> None:
+    """out_i = sum_m a_m sin(pi k_m0 x_i) sin(pi k_m1 y_i) sin(pi k_m2 z_i) at
+    the points coords [n, 3] (mfx_vec_sine_modes); wave_numbers: 3 integers
+    per mode, amplitudes: one per mode (at most 64 modes)."""
+    import numpy as np
+    k = np.ascontiguousarray(wave_numbers, dtype=np.int32)
+    a = np.ascontiguousarray(amplitudes, dtype=np.float64)
+    if k.size != 3 * a.size:
+        raise ValueError("three wave numbers per amplitude")
+    n = out.numel()
+    check(lib().mfx_vec_sine_modes(_ptr(coords, F64, 3 * n), n, a.size, k.ctypes.data,
+                                   a.ctypes.data, _ptr(out, F64, n), _stream()))
 
 
 def random_vector(n: int, seed: int) -> torch.Tensor:

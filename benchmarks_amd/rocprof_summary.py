@@ -9,6 +9,11 @@
       by what it reads for a kernel of known traffic in the same run: the
       deterministic dot (k_dot_partials, reads 16 N bytes) for FETCH_SIZE and
       the zero-fill (k_fill, writes 8 N bytes) for WRITE_SIZE.
+  python -m benchmarks_amd.rocprof_summary counters <kernel name part> <points>
+      <pmc counter_collection.csv> [...]
+      per-dispatch averages of the SQ counters of one kernel over one or more
+      `rocprofv3 --pmc` runs, and the ratios they give (wait share, FP64
+      instructions per quadrature point, LDS bank conflicts).
 """
 import csv
 import re
@@ -65,11 +70,67 @@ def traffic(fetch_csv, write_csv, n_dofs, title="HBM traffic"):
     return "\n".join(out)
 
 
+def counters(pmc_csvs, pattern, points=None, title="SQ counters"):
+    """Per-dispatch averages of every counter of the kernels whose name
+    contains `pattern`, over one or more `rocprofv3 --pmc` runs, with the
+    ratios that follow from those present: share of wave-cycles spent
+    waiting (SQ_WAIT_ANY / SQ_WAVE_CYCLES), FP64 VALU instructions per
+    quadrature point (wave instructions x 64 lanes / points), executed FP64
+    FLOPs (FMA = 2), LDS bank-conflict cycles per CU and the kernel's
+    register and LDS footprint."""
+    acc, meta = defaultdict(list), {}
+    for path in pmc_csvs:
+        for r in csv.DictReader(open(path)):
+            if pattern in r["Kernel_Name"]:
+                acc[r["Counter_Name"]].append(float(r["Counter_Value"]))
+                meta = {k: r[k] for k in ("Kernel_Name", "VGPR_Count", "Accum_VGPR_Count",
+                                          "SGPR_Count", "LDS_Block_Size", "Scratch_Size",
+                                          "Workgroup_Size", "Grid_Size") if k in r}
+    avg = {k: sum(v) / len(v) for k, v in acc.items()}
+    out = [f"### {title}", "", f"Kernel: `{short(meta.get('Kernel_Name', pattern))}`", "",
+           "| resource | value |", "|---|---:|"]
+    for k in ("VGPR_Count", "Accum_VGPR_Count", "SGPR_Count", "LDS_Block_Size", "Scratch_Size",
+              "Workgroup_Size", "Grid_Size"):
+        if k in meta:
+            out.append(f"| {k} | {meta[k]} |")
+    out += ["", "| counter | dispatches | per dispatch |", "|---|---:|---:|"]
+    for k in sorted(avg):
+        out.append(f"| {k} | {len(acc[k])} | {avg[k]:.6g} |")
+    out += ["", "| derived | value |", "|---|---:|"]
+    if "SQ_WAIT_ANY" in avg and avg.get("SQ_WAVE_CYCLES"):
+        out.append(f"| wave-cycles waiting (SQ_WAIT_ANY / SQ_WAVE_CYCLES) | "
+                   f"{avg['SQ_WAIT_ANY'] / avg['SQ_WAVE_CYCLES']:.3f} |")
+    f64 = [avg.get(f"SQ_INSTS_VALU_{op}_F64") for op in ("ADD", "MUL", "FMA")]
+    if all(v is not None for v in f64):
+        lanes = 64.0 * sum(f64)
+        flops = 64.0 * (f64[0] + f64[1] + 2.0 * f64[2])
+        out.append(f"| executed FP64 FLOPs per dispatch (FMA = 2) | {flops:.4e} |")
+        if points:
+            out.append(f"| FP64 lane instructions per quadrature point ({int(points)} points) | "
+                       f"{lanes / float(points):.1f} |")
+    if "SQ_LDS_BANK_CONFLICT" in avg:
+        out.append(f"| LDS bank-conflict cycles per CU (256 CUs) | "
+                   f"{avg['SQ_LDS_BANK_CONFLICT'] / 256:.4g} |")
+    if "SQ_LDS_BANK_CONFLICT" in avg and "SQ_LDS_IDX_ACTIVE" in avg:
+        out.append(f"| LDS bank-conflict share of LDS-active cycles | "
+                   f"{avg['SQ_LDS_BANK_CONFLICT'] / avg['SQ_LDS_IDX_ACTIVE']:.3f} |")
+    if "SQ_ACTIVE_INST_VALU" in avg and "GRBM_GUI_ACTIVE" in avg:
+        out.append(f"| VALU busy (SQ_ACTIVE_INST_VALU / CU / GRBM_GUI_ACTIVE) | "
+                   f"{avg['SQ_ACTIVE_INST_VALU'] / 256 / avg['GRBM_GUI_ACTIVE']:.3f} |")
+    if "SQ_INSTS_LDS" in avg and points:
+        out.append(f"| LDS instructions per quadrature point (x 64 lanes) | "
+                   f"{64.0 * avg['SQ_INSTS_LDS'] / float(points):.2f} |")
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
     mode = sys.argv[1]
     if mode == "stats":
         print(stats(sys.argv[2], *(sys.argv[3:4] or ["kernel trace"])))
     elif mode == "traffic":
         print(traffic(sys.argv[2], sys.argv[3], sys.argv[4], *(sys.argv[5:6] or ["HBM traffic"])))
+    elif mode == "counters":
+        # counters <pattern> <points> <pmc csv> [<pmc csv> ...]
+        print(counters(sys.argv[4:], sys.argv[2], float(sys.argv[3]) or None))
     else:
         raise SystemExit(__doc__)

@@ -237,6 +237,14 @@ mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b,
  * ---------------------------------------------------------------------- */
 /* Synthetic input: uniform[-1,1) from SplitMix64 with the given seed. */
 mfx_status mfx_vec_random(double *x, uint64_t n, uint64_t seed, void *stream);
+/* Smooth synthetic fields: out_i = sum_m a_m sin(pi k_m0 x_i) sin(pi k_m1 y_i)
+ * sin(pi k_m2 z_i) at points coords[n][3] (device), modes summed in index
+ * order; wave_numbers[3 n_modes] and amplitudes[n_modes] are host arrays,
+ * n_modes <= 64.  The right-hand sides of the learned preconditioner's
+ * training pairs. */
+mfx_status mfx_vec_sine_modes(const double *coords, uint64_t n, int n_modes,
+                              const int32_t *wave_numbers, const double *amplitudes, double *out,
+                              void *stream);
 /* Set the listed entries to zero (constrained rows of a right-hand side). */
 mfx_status mfx_vec_zero_constrained(mfx_op *op, double *x, void *stream);
 /* Deterministic dot product; result written to *host_result (synchronous). */
@@ -425,12 +433,33 @@ mfx_status mfx_nn_train_energy(mfx_nn *nn, mfx_op *op, int batch, const double *
 /* preconditioner application on a box operator's node grid (FP64 in/out,
  * constrained rows pass through) */
 mfx_status mfx_nn_vmult(mfx_nn *nn, mfx_op *op, double *z, const double *r, void *stream);
+/* The residual the two-level cycle hands the network, on the coarse
+ * operator's DoFs (FP64 out, rho_c of mfx_op_n_dofs(coarse) entries):
+ * rho_c = R (r - A z), z the cycle's FP32 Chebyshev pre-smoothing of A z = r
+ * from zero (constrained rows of r taken as 0, those of rho_c set to 0), R
+ * the transpose of the nodal interpolation.  Needs smoother degree > 0 and a
+ * coarse operator (mfx_nn_set_coarse).  Training pairs (rho_c,
+ * A_c^-1 rho_c) made from it are what the network meets in the solve. */
+mfx_status mfx_nn_restricted_residual(mfx_nn *nn, mfx_op *op, const double *r, double *rho_c,
+                                      void *stream);
 /* flexible CG (up to 4 stored A-orthogonal directions) on op preconditioned
  * by the network;
  * stopping rule and result fields of mfx_cg_solve */
 mfx_status mfx_nn_fcg_solve(mfx_nn *nn, mfx_op *op, double *x, const double *b, int max_steps,
                             double abs_tol, double rel_tol, mfx_cg_result *result,
                             void *stream);
+/* Fallback of mfx_nn_fcg_solve, the counterpart of the reference solver's
+ * only failure path (NoConvergence, bp5_kokkos/benchmark.cc:366-374): when
+ * the residual falls by less than a factor min_reduction over the last
+ * `window` steps, or the network's correction has been rejected (energy-
+ * optimal weight <= 0) in `window` applications in a row, or a direction
+ * breaks down, the solve continues with one V-cycle of `mg` per step (its
+ * fine operator must be the solve's operator; the caller keeps it alive).
+ * mg == NULL: no fallback.  window 0: 8; min_reduction 0: 2. */
+mfx_status mfx_nn_set_fallback(mfx_nn *nn, mfx_mg *mg, int window, double min_reduction);
+/* Step at which the last mfx_nn_fcg_solve switched to the fallback
+ * (-1: it did not). */
+int mfx_nn_fallback_step(const mfx_nn *nn);
 
 #ifdef __cplusplus
 }

@@ -377,6 +377,13 @@ class LearnedPreconditioner {
   void vmult(const LaplaceOperator<double> &A, Vector<double> &z, const Vector<double> &r) const {
     check(mfx_nn_vmult(nn_, A.handle(), z.get_values(), r.get_values(), nullptr));
   }
+  // the multigrid the flexible CG switches to when the learned cycle
+  // stagnates (mfx_nn_set_fallback); nullptr: none
+  void set_fallback(const PreconditionMG *mg, int window = 8, double min_reduction = 2.0) {
+    check(mfx_nn_set_fallback(nn_, mg ? mg->handle() : nullptr, window, min_reduction));
+  }
+  // step at which the last solve switched to the fallback, -1 if it did not
+  int fallback_step() const { return mfx_nn_fallback_step(nn_); }
   mfx_nn *handle() const { return nn_; }
 
  private:

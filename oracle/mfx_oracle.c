@@ -862,6 +862,10 @@ orc_cg_result orc_cg(int nm, int nq, const double *B, const double *D,
     res.converged = 1;
     goto done;
   }
+  if (max_steps <= 0 || isnan(rn)) { /* SolverControl::check fails at step 0 */
+    res.converged = 0;
+    goto done;
+  }
   double gamma_old = 0.0;
   for (int it = 1;; ++it) {
     for (uint64_t i = 0; i < N; ++i) v[i] = inv_diag ? inv_diag[i] * r[i] : r[i];

@@ -1,4 +1,6 @@
-"""CPU restatement (numpy, FP64) of the learned preconditioner's network.
+"""CPU restatement (numpy, FP64) of the learned preconditioner: its network,
+the two-level cycle it serves as coarse solve in, and the flexible CG
+around it (the last two at the end of this file).
 
 Test infrastructure only: tests/ import it, the product does not.
 
@@ -190,3 +192,95 @@ def init_params(seed: int) -> np.ndarray:
         out[k:k + m] = z[k:k + m] * np.sqrt(2.0 / (27 * ci))
         k += m
     return out
+
+
+# ---------------------------------------------------------------------------
+# The learned preconditioner as a solver component, composed from this
+# module's network and the multigrid oracle's Chebyshev smoother, Lanczos
+# estimate and transfers (oracle/mg_oracle.py).  No reference counterpart.
+# ---------------------------------------------------------------------------
+
+def smoother_interval(fine, iters: int = 12, safety: float = 1.2, smoothing_range: float = 15.0,
+                      seed: int = 0x5EED):
+    """[lambda_max / range, lambda_max] with lambda_max = safety x the
+    `iters`-step Lanczos estimate of the largest eigenvalue of D^-1 A, from
+    the SplitMix64 vector of `seed` with constrained rows zeroed: the
+    interval of the cycle's Chebyshev smoother."""
+    from . import mg_oracle as M
+    from . import oracle as O
+    r0 = O.splitmix64_uniform(seed, fine.N)
+    r0[fine.constrained.astype(bool)] = 0.0
+    lmax = safety * M.lanczos_lambda_max(fine.vmult, fine.inverse_diagonal(), r0, iters)
+    return lmax / smoothing_range, lmax
+
+
+def two_grid_apply(fine, coarse, P, params, out_scale, degree, lmin, lmax, r,
+                   bf16_storage=True, parts=False):
+    """FP64 restatement of the two-level cycle with the network as the coarse
+    solve (benchmarks_amd/csrc/nn.cpp, precondition_two_level with a coarse
+    operator): from rb = r with constrained rows 0,
+      z   = `degree` Chebyshev(Jacobi) steps on A z = rb from zero,
+      rho = rb - A z,   rho_c = P^T rho (coarse constrained rows 0),
+      c   = P (||rho_c|| s net(rho_c / ||rho_c||)) (constrained rows 0 on
+            both grids),
+      z  += omega c,  omega = c.rho / c.Ac (0 if c.Ac <= 0),
+      z   = `degree` Chebyshev steps from z; z_c = r_c on constrained rows.
+    fine, coarse: oracle Problems of one box; P: the prolongation between
+    them (mg_oracle.prolongation_3d); params: flat network weights.  Returns
+    (z, omega), or with parts=True also the iterates before and after the
+    correction."""
+    from . import mg_oracle as M
+    cons = fine.constrained.astype(bool)
+    cons_c = coarse.constrained.astype(bool)
+    dinv = fine.inverse_diagonal()
+    rb = np.array(r, dtype=np.float64)
+    rb[cons] = 0.0
+    z = M.chebyshev(fine.vmult, dinv, rb, None, degree, lmin, lmax)
+    z_pre = z.copy()
+    rho = rb - fine.vmult(z)
+    rc = P.T @ rho
+    rc[cons_c] = 0.0
+    rn = float(np.linalg.norm(rc))
+    omega = 0.0
+    if rn > 0.0:
+        cd = tuple(n * coarse.p + 1 for n in coarse.box.n)  # x, y, z node counts
+        y, _ = forward(split_params(params), (rc / rn).reshape(cd[::-1]), bf16_storage)
+        ec = rn * out_scale * y.ravel()
+        ec[cons_c] = 0.0
+        c = P @ ec
+        c[cons] = 0.0
+        cac = float(c @ fine.vmult(c))
+        omega = float(c @ rho) / cac if cac > 0.0 else 0.0
+        z = z + omega * c
+    z_corr = z.copy()
+    z = M.chebyshev(fine.vmult, dinv, rb, z, degree, lmin, lmax)
+    z[cons] = r[cons]
+    if parts:
+        return z, omega, z_pre, z_corr
+    return z, omega
+
+
+def fcg(vmult, precondition, b, max_steps, m: int = 4, x0=None):
+    """Flexible CG with up to m stored directions (Notay's FCG(m)) as
+    mfx_nn_fcg_solve runs it: z = precondition(r); p = z minus its
+    A-projections on the stored directions other than the slot it replaces,
+    p = z - sum_j (z.q_j / p_j.q_j) p_j with q_j = A p_j; alpha = p.r / p.q;
+    x += alpha p; r -= alpha q.  Returns (x, residual norms)."""
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+    r = b - vmult(x)
+    Ps, Qs, pq = [None] * m, [None] * m, [0.0] * m
+    hist = [float(np.linalg.norm(r))]
+    for steps in range(max_steps):
+        z = precondition(r)
+        slot, stored = steps % m, min(steps, m)
+        p = z.copy()
+        for j in range(stored):
+            if j != slot:
+                p -= (z @ Qs[j]) / pq[j] * Ps[j]
+        q = vmult(p)
+        Ps[slot], Qs[slot], pq[slot] = p, q, float(p @ q)
+        alpha = (p @ r) / pq[slot]
+        x += alpha * p
+        r -= alpha * q
+        hist.append(float(np.linalg.norm(r)))
+    return x, hist

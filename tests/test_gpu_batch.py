@@ -106,6 +106,33 @@ def test_batch_cg_on_the_fly_matches_oracle(gpu):
         assert np.linalg.norm(xh[a:e] - xo) <= 1e-7 * np.linalg.norm(xo), i
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("steps", [0, 1, 3])
+def test_batch_iterate_after_a_step_limit_matches_oracle(gpu, steps, use_graph):
+    """Every instance stopped by the step limit holds the oracle's iterate
+    after the same number of steps (the deferred x update of the last step is
+    applied when the batch stops); with max_steps = 0 no step is taken and x
+    keeps its start value, as deal.II's SolverControl fails at step 0."""
+    from benchmarks_amd.laplace import NoConvergence
+    B = _batch(SMALL)
+    b = _rhs(B, 8)
+    inv = B.compute_inverse_diagonal()
+    x = B.initialize_dof_vector()
+    with pytest.raises(NoConvergence):
+        B.solve(x, b, inv, max_steps=steps, abs_tol=1e-300, rel_tol=1e-14, use_graph=use_graph)
+    bh, xh = b.cpu().numpy(), x.cpu().numpy()
+    for i, (p, n) in enumerate(SMALL):
+        a, e = B.offsets[i], B.offsets[i + 1]
+        P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
+        xo, r, _ = P.cg(bh[a:e], inv_diag=P.inverse_diagonal(), max_steps=steps, abs_tol=1e-300,
+                        rel_tol=1e-14)
+        assert r.steps == steps and not r.converged
+        if steps == 0:
+            assert not xh[a:e].any(), i
+        else:
+            assert np.linalg.norm(xh[a:e] - xo) <= 1e-12 * np.linalg.norm(xo), i
+
+
 def test_batch_set_variant_rejects_bad_input(gpu):
     from benchmarks_amd._lib import MfxError
     B = _batch([(3, 2), (4, 2)])

@@ -130,6 +130,22 @@ def test_cg_max_steps_raises(gpu):
     assert ei.value.last_step == 5
 
 
+def test_cg_max_steps_zero_takes_no_step(gpu):
+    """max_steps = 0 on an unconverged start: NoConvergence at step 0 with x
+    unchanged (deal.II's SolverControl::check at step 0), as the oracle."""
+    from benchmarks_amd.laplace import (LaplaceOperator, NoConvergence, ReductionControl,
+                                        SolverCG)
+    op = LaplaceOperator(2, 3, O.GLL, n_cells=3, deform=0.05)
+    b = op.rhs(1)
+    x0 = torch.as_tensor(O.splitmix64_uniform(5, op.m()), device="cuda")
+    x = x0.clone()
+    ctl = ReductionControl(0, 1e-300, 1e-10)
+    with pytest.raises(NoConvergence) as ei:
+        SolverCG(ctl).solve(op, x, b, None)
+    assert ei.value.last_step == 0 and ctl.last_step() == 0
+    assert torch.equal(x, x0)
+
+
 def test_cg_zero_rhs_converges_at_step_zero(gpu):
     from benchmarks_amd.laplace import LaplaceOperator
     op = LaplaceOperator(2, 3, O.GLL, n_cells=3)

@@ -275,3 +275,228 @@ def test_flexible_cg_equals_pcg_for_linear_preconditioner(gpu):
     SolverCG(cj).solve(A, xj, b, DiagonalMatrix(A.compute_inverse_diagonal()))
     assert runs[0] == runs[1]
     assert runs[0] < cj.last_step()
+
+
+def test_coarse_network_on_two_fine_operators_in_turn(gpu):
+    """One network and coarse operator used with two fine operators of
+    different sizes one after the other, the first destroyed before the
+    second is made (so the second may reuse its address): the transfer and
+    its scratch are rebuilt for the second operator, and the application
+    equals a fresh network's on it (FP32 atomic sums: 1e-5)."""
+    import gc
+
+    import torch
+    from benchmarks_amd.laplace import LaplaceOperator, LearnedPreconditioner, random_vector
+    Cop = LaplaceOperator(1, 2, O.GLL, 4, deform=0.05, with_fp32=True)
+    net = LearnedPreconditioner(5)
+    net.set_coarse(Cop)
+    net.set_mode(2, True)
+    A1 = LaplaceOperator(2, 3, O.GLL, 4, deform=0.05, with_fp32=True)
+    r1 = random_vector(A1.m(), 3)
+    A1.zero_constrained(r1)
+    z1 = A1.initialize_dof_vector()
+    net.vmult(A1, z1, r1)
+    del A1, r1, z1
+    gc.collect()
+    A2 = LaplaceOperator(2, 3, O.GLL, 8, deform=0.05, with_fp32=True)
+    r2 = random_vector(A2.m(), 4)
+    A2.zero_constrained(r2)
+    z2 = A2.initialize_dof_vector()
+    net.vmult(A2, z2, r2)
+    fresh = LearnedPreconditioner(5)
+    fresh.set_coarse(Cop)
+    fresh.set_mode(2, True)
+    z3 = A2.initialize_dof_vector()
+    fresh.vmult(A2, z3, r2)
+    assert (torch.linalg.norm(z2 - z3) / torch.linalg.norm(z3)).item() <= 1e-5
+
+
+def _fallback_problem(n=8, p=2):
+    from benchmarks_amd.laplace import PreconditionMG, random_vector
+    mg = PreconditionMG(p, p + 1, O.GLL, n, deform=0.05, geometry=1)
+    A = mg.operator()
+    b = random_vector(A.m(), 12)
+    A.zero_constrained(b)
+    return mg, A, b
+
+
+@pytest.mark.parametrize("degree", [0, 3])
+def test_fallback_takes_over_from_a_dead_network(gpu, degree):
+    """A zeroed network (host-side switch: all weights 0) contributes
+    nothing.  Alone (smoother degree 0) it gives a zero direction, which
+    breaks flexible CG down at step 0; inside the two-grid cycle its
+    correction is rejected (omega = 0) at every application, and after
+    `window` of them the solve switches.  With the multigrid fallback set,
+    the solve converges in a few multigrid steps and reports the switch;
+    without it, degree 0 raises NoConvergence."""
+    from benchmarks_amd.laplace import (LearnedPreconditioner, NoConvergence, ReductionControl,
+                                        SolverCG)
+    mg, A, b = _fallback_problem()
+    net = LearnedPreconditioner(4)
+    net.set_params(np.zeros(net.n_params(), dtype=np.float32))
+    net.set_mode(degree, True)
+    if degree == 0:
+        with pytest.raises(NoConvergence):
+            SolverCG(ReductionControl(200, 1e-300, 1e-8)).solve(A, A.initialize_dof_vector(), b,
+                                                                net)
+    window = 4
+    net.set_fallback(mg, window=window)
+    x = A.initialize_dof_vector()
+    c = ReductionControl(200, 1e-300, 1e-8)
+    SolverCG(c).solve(A, x, b, net)
+    assert c.last_value() <= 1e-8 * c.initial_value()
+    xm = A.initialize_dof_vector()
+    cm = ReductionControl(200, 1e-300, 1e-8)
+    SolverCG(cm).solve(A, xm, b, mg)
+    expect = 0 if degree == 0 else window
+    assert net.fallback_step == expect, (net.fallback_step, expect)
+    assert c.last_step() <= expect + cm.last_step() + 2, (c.last_step(), cm.last_step())
+    # the FP64 residual of the returned solution meets the tolerance
+    r = A.initialize_dof_vector()
+    A.vmult(r, x)
+    assert (np.linalg.norm((b - r).cpu().numpy()) <= 1.01e-8 * np.linalg.norm(b.cpu().numpy()))
+
+
+def test_fallback_stays_off_for_a_working_preconditioner(gpu):
+    """The Chebyshev-only cycle (network off) reduces the residual steadily:
+    the stagnation test (less than 2x over the window) never fires, and the
+    rejected-correction test does not apply without the network."""
+    from benchmarks_amd.laplace import LearnedPreconditioner, ReductionControl, SolverCG
+    mg, A, b = _fallback_problem()
+    net = LearnedPreconditioner(4)
+    net.set_mode(3, False)
+    net.set_fallback(mg)
+    c = ReductionControl(2000, 1e-300, 1e-8)
+    SolverCG(c).solve(A, A.initialize_dof_vector(), b, net)
+    assert net.fallback_step == -1 and c.last_step() > 8
+
+
+def test_fallback_rejects_a_foreign_multigrid(gpu):
+    from benchmarks_amd._lib import MfxError
+    from benchmarks_amd.laplace import LearnedPreconditioner, ReductionControl, SolverCG
+    mg, A, b = _fallback_problem()
+    other, _, _ = _fallback_problem(n=6)
+    net = LearnedPreconditioner(4)
+    net.set_mode(2, False)
+    net.set_fallback(other)
+    with pytest.raises(MfxError):
+        SolverCG(ReductionControl(50, 1e-300, 1e-8)).solve(A, A.initialize_dof_vector(), b, net)
+
+
+def _cycle_setup(seed=5):
+    from benchmarks_amd.laplace import LaplaceOperator, LearnedPreconditioner
+    from oracle import mg_oracle as M
+    n, nc = (8, 6, 4), (4, 3, 2)
+    A = LaplaceOperator(2, 3, O.GLL, n, deform=0.05, with_fp32=True)
+    Cop = LaplaceOperator(1, 2, O.GLL, nc, deform=0.05, with_fp32=True)
+    fine = O.make_problem(2, 3, O.GLL, n, deform=0.05)
+    coarse = O.make_problem(1, 2, O.GLL, nc, deform=0.05)
+    net = LearnedPreconditioner(seed)
+    net.set_coarse(Cop)
+    return A, Cop, fine, coarse, M.prolongation_3d(fine, coarse), net
+
+
+def test_two_grid_cycle_matches_oracle(gpu):
+    """The library's two-level cycle (FP32 smoothing and transfers, network
+    on BF16 MFMA) against its FP64 restatement composed from the network and
+    multigrid oracles (nn_oracle.two_grid_apply, bfloat16 storage in the
+    network), same weights and right-hand side.  The Chebyshev interval
+    agrees to 1e-6 (both Lanczos estimates run in FP64); omega and z to 1e-2:
+    FP32 residuals move a few bfloat16 roundings of the network's
+    activations.  A dropped smoothing pass, transfer, weight or constraint
+    is an O(1) difference."""
+    A, Cop, fine, coarse, P, net = _cycle_setup()
+    deg = 2
+    net.set_mode(deg, True)
+    params = net.get_params().astype(np.float64)
+    r = O.splitmix64_uniform(17, A.m())
+    import torch
+    z = A.initialize_dof_vector()
+    net.vmult(A, z, torch.as_tensor(r, device="cuda"))
+    info = net.smoother_info(A)
+    lmin, lmax = N.smoother_interval(fine)
+    assert abs(info["lambda_max"] - lmax) <= 1e-6 * lmax
+    zo, omega = N.two_grid_apply(fine, coarse, P, params, net.output_scale, deg, lmin, lmax, r)
+    assert abs(info["omega"] - omega) <= 1e-2 * abs(omega), (info["omega"], omega)
+    assert _rel(z.cpu().numpy(), zo) <= 1e-2, _rel(z.cpu().numpy(), zo)
+
+
+@pytest.mark.parametrize("steps", [1, 2])
+def test_fcg_steps_match_oracle(gpu, steps):
+    """The first flexible-CG steps with the two-level cycle (mfx_nn_fcg_solve
+    stopped by the step limit) against the oracle's FCG(4) around the
+    oracle's cycle: the iterate to 1e-2 (the cycle's own agreement)."""
+    import torch
+    from benchmarks_amd.laplace import NoConvergence, ReductionControl, SolverCG
+    A, Cop, fine, coarse, P, net = _cycle_setup(6)
+    deg = 2
+    net.set_mode(deg, True)
+    params = net.get_params().astype(np.float64)
+    b = O.splitmix64_uniform(19, A.m())
+    b[fine.constrained.astype(bool)] = 0.0
+    x = A.initialize_dof_vector()
+    with pytest.raises(NoConvergence):
+        SolverCG(ReductionControl(steps, 1e-300, 1e-14)).solve(
+            A, x, torch.as_tensor(b, device="cuda"), net)
+    lmin, lmax = N.smoother_interval(fine)
+    s = net.output_scale
+    xo, hist = N.fcg(fine.vmult,
+                     lambda r: N.two_grid_apply(fine, coarse, P, params, s, deg, lmin, lmax, r)[0],
+                     b, steps)
+    assert hist[-1] < hist[0]
+    assert _rel(x.cpu().numpy(), xo) <= 1e-2, _rel(x.cpu().numpy(), xo)
+
+
+def test_learned_fcg_on_the_configured_problem(gpu):
+    """The learned-preconditioner configuration at its size: Q4 BP5 on 64^3
+    deformed cells (16,974,593 DoFs), the network as the coarse solve on the
+    Q1 grid of 32^3 cells (33^3 nodes), trained on 32 pairs from the FP64
+    multigrid CG of that grid (half restricted pre-smoothed fine residuals;
+    the bench trains on 4096), smoother degree 4, a smooth right-hand side.
+    Flexible CG to 1e-8 with the multigrid fallback armed: the residual
+    recomputed in FP64 from the returned solution, by the library and by the
+    oracle's CPU operator, meets 1e-8, the fallback
+    never engages, the trained network takes fewer steps than the same cycle
+    without it, and fewer fine-operator applications (2k + 2 per FCG step:
+    2k Chebyshev, one for omega, FCG's own) than Jacobi-CG (one per step)."""
+    import torch
+    from benchmarks_amd.laplace import (DiagonalMatrix, LearnedPreconditioner, PreconditionMG,
+                                        ReductionControl, SolverCG)
+    from benchmarks_amd.learned import _stream, smooth_rhs, train_coarse
+    n, deg = 64, 4
+    mg = PreconditionMG(4, 5, O.GLL, n, deform=0.05, geometry=1)
+    A = mg.operator()
+    assert A.m() == 16974593
+    cmg = PreconditionMG(1, 2, O.GLL, n // 2, deform=0.05)
+    net = LearnedPreconditioner(3)
+    st = train_coarse(net, cmg, n_fields=32, mse_steps=200, energy_steps=600, seed=3, fine=A,
+                      smoother_degree=deg)
+    assert st["coarse_dims"] == (33, 33, 33) and st["data_solver"] == "multigrid CG"
+    net.set_mode(deg, True)
+    net.set_fallback(mg)
+    b = smooth_rhs(A, A.dof_coordinates(), _stream(4), 32)
+    steps = {}
+    without = LearnedPreconditioner(3)
+    without.set_mode(deg, False)
+    for name, P in [("learned", net), ("without", without),
+                    ("jacobi", DiagonalMatrix(A.compute_inverse_diagonal()))]:
+        x = A.initialize_dof_vector()
+        c = ReductionControl(20000, 1e-300, 1e-8)
+        SolverCG(c).solve(A, x, b, P)
+        steps[name] = c.last_step()
+        if name == "learned":
+            r = A.initialize_dof_vector()
+            A.vmult(r, x)
+            res = (torch.linalg.norm(b - r) / torch.linalg.norm(b)).item()
+            assert res <= 1e-8, res
+            assert net.fallback_step == -1
+            # the same certificate from the oracle's own operator on the CPU
+            # (its own mesh, map and stored metric terms)
+            Po = O.make_problem(4, 5, O.GLL, n, deform=0.05)
+            xh, bh = x.cpu().numpy(), b.cpu().numpy()
+            res_o = np.linalg.norm(bh - Po.vmult(xh, threads=16)) / np.linalg.norm(bh)
+            assert res_o <= 1.01e-8, res_o
+            del Po
+    print("C5 steps", steps)
+    assert steps["learned"] < steps["without"], steps
+    assert (2 * deg + 2) * steps["learned"] < steps["jacobi"], steps

@@ -52,3 +52,15 @@ def test_manufactured_solution_convergence(p):
         errs.append(P.l2_error_sin(x))
     rate = np.log2(errs[0] / errs[1])
     assert rate > p + 0.5, (errs, rate)
+
+
+def test_cg_max_steps_zero_fails_at_step_zero():
+    """deal.II's SolverControl::check fails at step 0 when max_steps is 0 and
+    the start is not converged: no step is taken and x keeps its value."""
+    P = O.make_problem(2, 3, O.GLL, 3)
+    b = P.rhs(1)
+    x0 = O.splitmix64_uniform(5, P.N)
+    x, r, hist = P.cg(b, rel_tol=1e-10, max_steps=0, x0=x0)
+    assert not r.converged and r.steps == 0
+    assert np.array_equal(x, x0)
+    assert len(hist) == 1

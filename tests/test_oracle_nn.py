@@ -94,3 +94,107 @@ def test_bf16_rounding_is_nearest_even():
     assert np.array_equal(N.bf16(v), want)
     ties = np.array([1 + 2.0 ** -8, 1 + 3 * 2.0 ** -8, -(1 + 2.0 ** -8), 2.0 ** -8 * (1 + 2.0 ** -8)])
     assert np.array_equal(N.bf16(ties), [1.0, 1 + 2.0 ** -6, -1.0, 2.0 ** -8])
+
+
+# --- the two-level cycle and flexible CG (nn_oracle.two_grid_apply, fcg) ---
+
+def _two_grid_problem():
+    """Q2 on 4^3 deformed cells with the Q1 grid of 2^3 cells as coarse grid
+    (one free coarse DoF); small enough for dense checks."""
+    from oracle import mg_oracle as M
+    from oracle import oracle as O
+    fine = O.make_problem(2, 3, O.GLL, 4, deform=0.05)
+    coarse = O.make_problem(1, 2, O.GLL, 2, deform=0.05)
+    return fine, coarse, M.prolongation_3d(fine, coarse)
+
+
+def _dense(prob):
+    free = np.flatnonzero(prob.constrained == 0)
+    A = np.zeros((free.size, free.size))
+    for j, dof in enumerate(free):
+        e = np.zeros(prob.N)
+        e[dof] = 1.0
+        A[:, j] = prob.vmult(e)[free]
+    return free, 0.5 * (A + A.T)
+
+
+def test_two_grid_cycle_without_network_is_the_squared_chebyshev_polynomial():
+    """With all weights zero the network adds nothing (omega = 0) and the
+    cycle is k Chebyshev steps from zero followed by k more from that
+    iterate: the error A^-1 r - z is T(M)^2 A^-1 r, M = D^-1 A and T the
+    scaled Chebyshev polynomial, evaluated through the eigenvectors of
+    D^-1/2 A D^-1/2 (a dropped smoothing pass, a wrong interval or a
+    misplaced Jacobi scaling fails this)."""
+    from oracle import mg_oracle as M
+    fine, coarse, P = _two_grid_problem()
+    lmin, lmax = N.smoother_interval(fine)
+    r = np.random.default_rng(3).standard_normal(fine.N)
+    k = 3
+    z, omega = N.two_grid_apply(fine, coarse, P, np.zeros(N.n_params()), 1.0, k, lmin, lmax, r)
+    assert omega == 0.0
+    free, A = _dense(fine)
+    d = np.diag(A)
+    S = A / np.sqrt(np.outer(d, d))
+    lam, V = np.linalg.eigh(S)
+    T = M.chebyshev_error_polynomial(lam, k, lmin, lmax)
+    e0 = np.linalg.solve(A, r[free])
+    e2 = (V * T ** 2) @ (V.T @ (np.sqrt(d) * e0)) / np.sqrt(d)
+    assert np.linalg.norm((e0 - z[free]) - e2) <= 1e-11 * np.linalg.norm(e0)
+    cons = fine.constrained.astype(bool)
+    assert np.array_equal(z[cons], r[cons])
+
+
+def test_two_grid_correction_is_energy_optimal():
+    """With a nonzero network the correction omega c lowers the energy norm
+    of the error, and omega is its minimiser along c: moving it by 10% either
+    way raises the energy."""
+    fine, coarse, P = _two_grid_problem()
+    lmin, lmax = N.smoother_interval(fine)
+    r = np.random.default_rng(4).standard_normal(fine.N)
+    r[fine.constrained.astype(bool)] = 0.0
+    params = N.init_params(5)
+    z, omega, z_pre, z_corr = N.two_grid_apply(fine, coarse, P, params, 0.05, 2, lmin, lmax, r,
+                                               parts=True)
+    free, A = _dense(fine)
+    e0 = np.zeros(fine.N)
+    e0[free] = np.linalg.solve(A, r[free])
+
+    def energy(x):
+        e = (e0 - x)[free]
+        return e @ A @ e
+    assert omega != 0.0
+    assert energy(z_corr) < energy(z_pre)
+    c = (z_corr - z_pre) / omega
+    for f in (0.9, 1.1):
+        assert energy(z_pre + f * omega * c) > energy(z_corr)
+
+
+def test_two_grid_cycle_is_positively_homogeneous():
+    """The network sees r / ||r|| and its output is scaled by ||r||; the rest
+    of the cycle is linear: cycle(2 r) = 2 cycle(r)."""
+    fine, coarse, P = _two_grid_problem()
+    lmin, lmax = N.smoother_interval(fine)
+    r = np.random.default_rng(6).standard_normal(fine.N)
+    params = N.init_params(7)
+    z1, _ = N.two_grid_apply(fine, coarse, P, params, 0.05, 2, lmin, lmax, r)
+    z2, _ = N.two_grid_apply(fine, coarse, P, params, 0.05, 2, lmin, lmax, 2.0 * r)
+    assert np.linalg.norm(z2 - 2.0 * z1) <= 1e-13 * np.linalg.norm(z1)
+
+
+@pytest.mark.parametrize("m", [2, 4])
+def test_fcg_with_a_linear_preconditioner_is_pcg(m):
+    """With a fixed SPD preconditioner (Jacobi) flexible CG is PCG in exact
+    arithmetic once it keeps the previous direction (m >= 2: the slot being
+    replaced is not projected out): its iterates match the oracle's
+    deal.II-semantics CG (orc_cg) step by step."""
+    from oracle import oracle as O
+    prob = O.make_problem(3, 4, O.GLL, 3, deform=0.05)
+    b = O.splitmix64_uniform(8, prob.N)
+    b[prob.constrained.astype(bool)] = 0.0
+    dinv = prob.inverse_diagonal()
+    for steps in (1, 2, 6):
+        x, hist = N.fcg(prob.vmult, lambda r: dinv * r, b, steps, m=m)
+        xo, res, ho = prob.cg(b, inv_diag=dinv, max_steps=steps, abs_tol=1e-300, rel_tol=1e-14)
+        assert res.steps == steps
+        assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo), (m, steps)
+        assert np.allclose(hist, ho, rtol=1e-9)
