@@ -26,6 +26,7 @@
 #include <type_traits>
 
 #include "mfx_internal.hpp"
+#include "tables.hpp"
 
 namespace mfx {
 namespace {
@@ -184,20 +185,34 @@ k_apply_colloc(const T *__restrict__ gD, const T *__restrict__ G,
 // General tensor-product element (BK3): NM = p+1 nodes, NQ >= NM quadrature
 // points, interpolation B[q][m] and collocation derivative D[q][k].
 // ---------------------------------------------------------------------------
-template <typename T, int NM, int NQ, bool GATHER, bool DOT>
+//
+// OTF: the metric terms are computed per quadrature point from the 8
+// vertices of the trilinear cell (cellX [E][8][3], Gauss points gxq and
+// weights gwq on [0,1]) instead of being read from G: per cell seven
+// coefficient vectors of the Jacobian (LDS), per (x,y) column its Jacobian
+// columns as functions of zeta, per point the cofactor columns
+// C0 = J1 x J2, C1 = J2 x J0, C2 = J0 x J1 (rows of det J J^-1), det J =
+// J2 . C2 and (G grad u)_a = (w / det J) C_a . sum_b (du/dxi_b) C_b, with
+// G = w det J J^-1 J^-T the stored form.  24 doubles per cell replace
+// 48 nq^3 bytes of G.
+template <typename T, int NM, int NQ, bool GATHER, bool DOT, bool OTF = false>
 __global__ void __launch_bounds__(Tile<NQ>::THREADS)
 k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
                const T *__restrict__ G, const T *__restrict__ src,
                T *__restrict__ dst, const uint32_t *__restrict__ map,
                uint32_t n_cells, double *__restrict__ partials,
-               const int *__restrict__ skip) {
+               const int *__restrict__ skip, const double *__restrict__ cellX,
+               const T *__restrict__ gxq, const T *__restrict__ gwq) {
   constexpr int EPB = Tile<NQ>::EPB, NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
   constexpr int NM2 = NM * NM, NM3 = NM2 * NM;
+  constexpr int THREADS = Tile<NQ>::THREADS;
   __shared__ T sB[NQ * NM];
   __shared__ T sD[NQ * NQ];
   __shared__ T s0[EPB][NQ3];
   __shared__ T s1[EPB][NQ3];
   __shared__ T s2[EPB][NQ3];
+  __shared__ T sXv[OTF ? EPB * 24 : 1];  // cell vertices, then
+  __shared__ T sKc[OTF ? EPB * 21 : 1];  // per cell 7 Jacobian coefficient vectors
   __shared__ double red[(Tile<NQ>::THREADS + 63) / 64];
 
   if (skip && *skip) return;  // converged CG: nothing left to do
@@ -224,9 +239,15 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   // registers after the x interpolation, so their stream overlaps the y/z
   // interpolation (3-20% faster at p = 7, 8 with q = 9, 10; slower at nq = 7
   // in FP64, no gain below).  Smaller nq loads them in the gradient phase.
-  constexpr bool PF = NQ >= 9;
-  const T *Ge = G + (size_t)(active ? e : 0) * 6 * NQ3 + col;
+  constexpr bool PF = NQ >= 9 && !OTF;
+  const T *Ge = OTF ? nullptr : G + (size_t)(active ? e : 0) * 6 * NQ3 + col;
   T g[PF ? 6 : 1][PF ? NQ : 1];
+  if constexpr (OTF) {
+    for (int i = tid; i < EPB * 24; i += THREADS) {
+      const uint32_t ec = blockIdx.x * EPB + i / 24;
+      sXv[i] = ec < n_cells ? (T)cellX[(size_t)ec * 24 + i % 24] : T(0);
+    }
+  }
 
   // 1. gather the (x,y) node column
   uint32_t idx[NM];
@@ -249,6 +270,25 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   }
   __syncthreads();
 
+  // per cell the Jacobian coefficient vectors (vertex v = a + 2b + 4c from
+  // the bits of (xi, eta, zeta)): A0 = x1-x0, A1 = x3-x2-x1+x0, G0 = x2-x0,
+  // K0 = x4-x0, K1 = x5-x1-x4+x0, K2 = x6-x2-x4+x0, K3 = x7-x6-x5+x4-x3+x2+x1-x0
+  if constexpr (OTF) {
+    for (int j = tid; j < EPB * 3; j += THREADS) {  // j = 3 cell + coordinate
+      const T *X = sXv + (j / 3) * 24 + j % 3;
+      const T x0 = X[0], x1 = X[3], x2 = X[6], x3 = X[9];
+      const T x4 = X[12], x5 = X[15], x6 = X[18], x7 = X[21];
+      T *Kd = sKc + (j / 3) * 21 + 7 * (j % 3);
+      const T a0 = x1 - x0, a1 = (x3 - x2) - a0, k0 = x4 - x0, k1 = (x5 - x1) - k0;
+      Kd[0] = a0;
+      Kd[1] = a1;
+      Kd[2] = x2 - x0;
+      Kd[3] = k0;
+      Kd[4] = k1;
+      Kd[5] = (x6 - x2) - k0;
+      Kd[6] = ((x7 - x6) - (x5 - x4)) - a1;
+    }
+  }
   // 2. interpolate along x: s1[z][y][qx], z,y < NM
   if (active && qy < NM) {
     T bx[NM];  // this lane's row of B, read from LDS once
@@ -265,6 +305,23 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   __syncthreads();
 
   // 3./4. interpolate along y (LDS) and z (registers), then 5. gradient
+  // OTF: the column's Jacobian columns J0 = a0 + zeta da, J1 = b0 + zeta db,
+  // J2 = jz at (xi, eta) = (gxq[qx], gxq[qy])
+  T ca0[3], cda[3], cb0[3], cdb[3], cjz[3];
+  if (OTF && active) {
+    const T xi = gxq[qx], eta = gxq[qy];
+    const T *K = sKc + el * 21;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T A0 = K[7 * d], A1 = K[7 * d + 1], G0 = K[7 * d + 2], K0 = K[7 * d + 3];
+      const T K1 = K[7 * d + 4], K2 = K[7 * d + 5], K3 = K[7 * d + 6];
+      ca0[d] = A0 + eta * A1;
+      cda[d] = K1 + eta * K3;
+      cb0[d] = G0 + xi * A1;
+      cdb[d] = K2 + xi * K3;
+      cjz[d] = (K0 + xi * K1) + eta * cdb[d];
+    }
+  }
   T uq[NQ];
   if (PF && active) {
 #pragma unroll
@@ -302,20 +359,9 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       dxr[a] = sD[qx * NQ + a];
       dyr[a] = sD[qy * NQ + a];
     }
+    const T wxy = OTF ? gwq[qx] * gwq[qy] : T(0);
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
-      T gxx, gxy, gxz, gyy, gyz, gzz;
-      if constexpr (PF) {
-        gxx = g[0][z]; gxy = g[1][z]; gxz = g[2][z];
-        gyy = g[3][z]; gyz = g[4][z]; gzz = g[5][z];
-      } else {
-        gxx = __builtin_nontemporal_load(Ge + 0 * NQ3 + z * NQ2);
-        gxy = __builtin_nontemporal_load(Ge + 1 * NQ3 + z * NQ2);
-        gxz = __builtin_nontemporal_load(Ge + 2 * NQ3 + z * NQ2);
-        gyy = __builtin_nontemporal_load(Ge + 3 * NQ3 + z * NQ2);
-        gyz = __builtin_nontemporal_load(Ge + 4 * NQ3 + z * NQ2);
-        gzz = __builtin_nontemporal_load(Ge + 5 * NQ3 + z * NQ2);
-      }
       T dx = 0, dy = 0, dz = 0;
 #pragma unroll
       for (int a = 0; a < NQ; ++a) {
@@ -323,9 +369,47 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
         dy += dyr[a] * s2[el][z * NQ2 + a * NQ + qx];
         dz += uD(z, a) * uq[a];
       }
-      s0[el][z * NQ2 + col] = gxx * dx + gxy * dy + gxz * dz;
-      s1[el][z * NQ2 + col] = gxy * dx + gyy * dy + gyz * dz;
-      fz[z] = gxz * dx + gyz * dy + gzz * dz;
+      if constexpr (OTF) {
+        const T zeta = gxq[z];
+        T J0[3], J1[3], C0[3], C1[3], C2[3], W[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          J0[d] = ca0[d] + zeta * cda[d];
+          J1[d] = cb0[d] + zeta * cdb[d];
+        }
+        C0[0] = J1[1] * cjz[2] - J1[2] * cjz[1];
+        C0[1] = J1[2] * cjz[0] - J1[0] * cjz[2];
+        C0[2] = J1[0] * cjz[1] - J1[1] * cjz[0];
+        C1[0] = cjz[1] * J0[2] - cjz[2] * J0[1];
+        C1[1] = cjz[2] * J0[0] - cjz[0] * J0[2];
+        C1[2] = cjz[0] * J0[1] - cjz[1] * J0[0];
+        C2[0] = J0[1] * J1[2] - J0[2] * J1[1];
+        C2[1] = J0[2] * J1[0] - J0[0] * J1[2];
+        C2[2] = J0[0] * J1[1] - J0[1] * J1[0];
+        const T det = cjz[0] * C2[0] + cjz[1] * C2[1] + cjz[2] * C2[2];
+        const T sc = wxy * gwq[z] * recip(det);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) W[d] = dx * C0[d] + dy * C1[d] + dz * C2[d];
+        s0[el][z * NQ2 + col] = sc * (C0[0] * W[0] + C0[1] * W[1] + C0[2] * W[2]);
+        s1[el][z * NQ2 + col] = sc * (C1[0] * W[0] + C1[1] * W[1] + C1[2] * W[2]);
+        fz[z] = sc * (C2[0] * W[0] + C2[1] * W[1] + C2[2] * W[2]);
+      } else {
+        T gxx, gxy, gxz, gyy, gyz, gzz;
+        if constexpr (PF) {
+          gxx = g[0][z]; gxy = g[1][z]; gxz = g[2][z];
+          gyy = g[3][z]; gyz = g[4][z]; gzz = g[5][z];
+        } else {
+          gxx = __builtin_nontemporal_load(Ge + 0 * NQ3 + z * NQ2);
+          gxy = __builtin_nontemporal_load(Ge + 1 * NQ3 + z * NQ2);
+          gxz = __builtin_nontemporal_load(Ge + 2 * NQ3 + z * NQ2);
+          gyy = __builtin_nontemporal_load(Ge + 3 * NQ3 + z * NQ2);
+          gyz = __builtin_nontemporal_load(Ge + 4 * NQ3 + z * NQ2);
+          gzz = __builtin_nontemporal_load(Ge + 5 * NQ3 + z * NQ2);
+        }
+        s0[el][z * NQ2 + col] = gxx * dx + gxy * dy + gxz * dz;
+        s1[el][z * NQ2 + col] = gxy * dx + gyy * dy + gyz * dz;
+        fz[z] = gxz * dx + gyz * dy + gzz * dz;
+      }
     }
   }
   __syncthreads();
@@ -431,10 +515,12 @@ void launch_t(const T *B, const T *D, const T *G, const T *src, T *dst,
     if (partials) {
       if constexpr (std::is_same<T, double>::value && GATHER)
         hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, true>), grid, block, 0, s,
-                           B, D, G, src, dst, map, n_cells, partials, skip);
+                           B, D, G, src, dst, map, n_cells, partials, skip, nullptr, nullptr,
+                           nullptr);
     } else {
       hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, false>), grid, block, 0, s,
-                         B, D, G, src, dst, map, n_cells, nullptr, nullptr);
+                         B, D, G, src, dst, map, n_cells, nullptr, nullptr, nullptr, nullptr,
+                         nullptr);
     }
   }
   MFX_HIP(hipGetLastError());
@@ -546,6 +632,60 @@ void launch_evec(int p, int nq, bool colloc, const T *B, const T *D, const T *G,
                  const T *in, T *out, uint32_t n_cells, hipStream_t s) {
   dispatch<T, false>(p, nq, colloc, B, D, G, in, out, nullptr, n_cells, s, nullptr, nullptr);
 }
+
+// Gauss (non-collocated) apply with the metric terms computed from the cell
+// vertices (k_apply_interp<..., OTF>).
+template <typename T, int NM, int NQ>
+void launch_interp_otf(const T *B, const T *D, const T *xq, const T *wq, const double *cellX,
+                       const T *src, T *dst, const uint32_t *map, uint32_t n_cells,
+                       hipStream_t s, double *partials, const int *skip) {
+  if (n_cells == 0) return;
+  const dim3 grid(n_blocks_for<NQ, false>(n_cells)), block(Tile<NQ>::THREADS);
+  if (partials) {
+    if constexpr (std::is_same<T, double>::value)
+      hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, true, true>), grid, block, 0, s, B, D,
+                         nullptr, src, dst, map, n_cells, partials, skip, cellX, xq, wq);
+    else
+      throw Error{MFX_ERR_UNSUPPORTED, "p.Ap partials are computed in FP64 only"};
+  } else {
+    hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, false, true>), grid, block, 0, s, B, D,
+                       nullptr, src, dst, map, n_cells, nullptr, nullptr, cellX, xq, wq);
+  }
+  MFX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *xq, const T *wq,
+                      const double *cellX, const T *src, T *dst, const uint32_t *map,
+                      uint32_t n_cells, hipStream_t s, double *partials, const int *skip) {
+  MFX_REQUIRE(B && D && xq && wq && cellX, "on-the-fly apply: missing tables or vertices");
+#define MFX_CASE_O(P)                                                                         \
+  case P:                                                                                    \
+    if (nq == P + 1)                                                                         \
+      return launch_interp_otf<T, P + 1, P + 1>(B, D, xq, wq, cellX, src, dst, map, n_cells, \
+                                                s, partials, skip);                          \
+    if (nq == P + 2)                                                                         \
+      return launch_interp_otf<T, P + 1, P + 2>(B, D, xq, wq, cellX, src, dst, map, n_cells, \
+                                                s, partials, skip);                          \
+    break;
+  switch (p) {
+    MFX_CASE_O(1) MFX_CASE_O(2) MFX_CASE_O(3) MFX_CASE_O(4)
+    MFX_CASE_O(5) MFX_CASE_O(6) MFX_CASE_O(7) MFX_CASE_O(8) default: break;
+  }
+#undef MFX_CASE_O
+  throw Error{MFX_ERR_UNSUPPORTED,
+              "unsupported (degree, n_q_points_1d) = (" + std::to_string(p) + ", " +
+                  std::to_string(nq) + ")"};
+}
+
+template void launch_apply_otf<double>(int, int, const double *, const double *, const double *,
+                                       const double *, const double *, const double *, double *,
+                                       const uint32_t *, uint32_t, hipStream_t, double *,
+                                       const int *);
+template void launch_apply_otf<float>(int, int, const float *, const float *, const float *,
+                                      const float *, const double *, const float *, float *,
+                                      const uint32_t *, uint32_t, hipStream_t, double *,
+                                      const int *);
 
 template void launch_apply<double>(int, int, bool, const double *, const double *,
                                    const double *, const double *, double *,

@@ -93,6 +93,71 @@ static void lexicographic_G(const mfx_op *op, GeometryView &v, hipStream_t s) {
   v.G = v.tmp.ptr;
 }
 
+namespace mfx {
+
+size_t cg_partial_count(const mfx_op &op) {
+  const uint32_t nb_apply = op.apply_blocks();
+  const uint32_t nb_cons = cg_cons_blocks(op.n_cons);
+  const uint32_t nb_vec = cg_vec_blocks(op.N);
+  return std::max<size_t>((size_t)nb_apply + nb_cons, 2 * (size_t)nb_vec);
+}
+
+// deal.II SolverCG + ReductionControl on the device (cg_state.hpp): the
+// step kernels are enqueued in batches of 4, 8, ... 64 and the host reads
+// the `done` flag once per batch.
+void cg_run(mfx_op &op, double *x, const double *b, const double *inv_diag, int max_steps,
+            double abs_tol, double rel_tol, const CgWork &w, mfx_cg_result *res, hipStream_t s) {
+  const uint64_t N = op.N;
+  const uint32_t nb_apply = op.apply_blocks();
+  const uint32_t nb_cons = cg_cons_blocks(op.n_cons);
+  const uint32_t nb_vec = cg_vec_blocks(N);
+  CgState init{};
+  MFX_HIP(hipMemcpyAsync(w.st, &init, sizeof(CgState), hipMemcpyHostToDevice, s));
+
+  const auto t0 = std::chrono::steady_clock::now();
+  // r = A x
+  launch_fill(w.r, 0.0, N, s);
+  op.apply64(x, w.r, s);
+  launch_copy_listed<double>(w.r, x, op.cons.ptr, op.n_cons, s);
+  cg_init(w.st, b, w.r, inv_diag, N, w.part, abs_tol, rel_tol, max_steps, s);
+  const int *done = &w.st->done;
+  int h_local = 0;
+  int *h_done = w.h_done ? w.h_done : &h_local;
+  int batch = 4;
+  for (int it = 0;;) {
+    for (int k = 0; k < batch; ++k, ++it) {
+      cg_direction(w.st, w.r, inv_diag, w.p, w.ap, x, N, s);
+      op.apply64(w.p, w.ap, s, w.part, done);
+      cg_constrained(w.st, op.cons.ptr, op.n_cons, w.p, w.ap, w.part + nb_apply, s);
+      cg_alpha(w.st, w.part, nb_apply + nb_cons, s);
+      cg_update(w.st, w.r, w.ap, inv_diag, N, w.part, s);
+      cg_check(w.st, w.part, nb_vec, s);
+    }
+    MFX_HIP(hipGetLastError());
+    MFX_HIP(hipMemcpyAsync(h_done, done, sizeof(int), hipMemcpyDeviceToHost, s));
+    MFX_HIP(hipStreamSynchronize(s));
+    if (*h_done) break;
+    if (it > max_steps + 8) break;  // device enforces max_steps; this bounds the host loop
+    batch = std::min(batch * 2, 64);
+  }
+  cg_finish(w.st, x, w.p, N, s);
+  CgState fin{};
+  MFX_HIP(hipMemcpyAsync(&fin, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+  MFX_HIP(hipStreamSynchronize(s));
+  const auto t1 = std::chrono::steady_clock::now();
+  res->steps = fin.steps;
+  res->converged = fin.done == 1;
+  res->initial_value = fin.initial;
+  res->last_value = fin.last;
+  res->seconds = std::chrono::duration<double>(t1 - t0).count();
+  if (fin.done != 1)
+    throw Error{fin.breakdown ? MFX_ERR_BREAKDOWN : MFX_ERR_NOT_CONVERGED,
+                "CG did not converge: steps=" + std::to_string(fin.steps) +
+                    " residual=" + std::to_string(fin.last)};
+}
+
+}  // namespace mfx
+
 extern "C" {
 
 const char *mfx_last_error(void) { return g_last_error.c_str(); }
@@ -180,8 +245,6 @@ mfx_status mfx_op_create_box(const mfx_box_desc *d, mfx_op **out) {
     *out = nullptr;
     auto op = std::make_unique<mfx_op>();
     common_init(op.get(), d->degree, d->n_q_points_1d, d->quadrature, d->geometry);
-    MFX_REQUIRE(d->geometry == MFX_GEOM_STORED || op->colloc,
-                "on-the-fly geometry is implemented for GLL collocation (nq = p+1)");
     const int p = d->degree;
     uint64_t E = 1, N = 1, Ninner = 1;
     for (int k = 0; k < 3; ++k) {
@@ -223,8 +286,10 @@ mfx_status mfx_op_create_box(const mfx_box_desc *d, mfx_op **out) {
       }
     op->n_cons = cons.size();
     op->cons.upload(cons.data(), cons.size());
-    // stored metric terms stream through the per-cell kernel (measured faster
-    // with G); on-the-fly geometry runs the patch kernel
+    // stored metric terms stream through the per-cell kernels (measured faster
+    // with G); on-the-fly geometry runs the patch kernel for GLL collocation
+    // and the per-cell Gauss kernel with the metric from the cell vertices
+    // otherwise
     if (op->colloc && d->geometry == MFX_GEOM_ON_THE_FLY) {
       op->patch = true;
       op->shape = patch_shape(p);
@@ -251,10 +316,11 @@ mfx_status mfx_op_create(const mfx_op_desc *d, mfx_op **out) {
     MFX_REQUIRE(d->map && d->cell_vertices, "map and cell_vertices are required");
     MFX_REQUIRE(d->n_constrained == 0 || d->constrained, "constrained list missing");
     MFX_REQUIRE(d->n_dofs < 0x7FFFFFFFull, "too many DoFs for 31-bit DoF indices");
-    MFX_REQUIRE(d->geometry == MFX_GEOM_STORED,
-                "general meshes use stored geometry (on-the-fly needs the box generator)");
     auto op = std::make_unique<mfx_op>();
     common_init(op.get(), d->degree, d->n_q_points_1d, d->quadrature, d->geometry);
+    MFX_REQUIRE(d->geometry == MFX_GEOM_STORED || !op->colloc,
+                "general meshes with GLL collocation use stored geometry (the on-the-fly "
+                "collocation kernel works on patches of a box mesh)");
     const uint64_t nm3 = (uint64_t)(d->degree + 1) * (d->degree + 1) * (d->degree + 1);
     const uint64_t nmap = (uint64_t)d->n_cells * nm3;
     for (uint64_t i = 0; i < nmap; ++i)
@@ -387,12 +453,8 @@ mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b, const double *in
   return guard([&] {
     MFX_REQUIRE(op && x && b && res, "null pointer argument");
     MFX_REQUIRE(max_steps >= 0, "max_steps must be >= 0");
-    hipStream_t s = as_stream(stream);
     const uint64_t N = op->N;
-    const uint32_t nb_apply = op->apply_blocks();
-    const uint32_t nb_cons = cg_cons_blocks(op->n_cons);
-    const uint32_t nb_vec = cg_vec_blocks(N);
-    const size_t npart = std::max<size_t>((size_t)nb_apply + nb_cons, 2 * (size_t)nb_vec);
+    const size_t npart = cg_partial_count(*op);
     if (op->r.n != N) {
       op->r.alloc(N);
       op->pdir.alloc(N);
@@ -401,48 +463,8 @@ mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b, const double *in
     if (op->part.n < npart) op->part.alloc(npart);
     if (!op->st.ptr) op->st.alloc(1);
     if (!op->h_done) MFX_HIP(hipHostMalloc(reinterpret_cast<void **>(&op->h_done), sizeof(int)));
-    CgState init{};
-    MFX_HIP(hipMemcpyAsync(op->st.ptr, &init, sizeof(CgState), hipMemcpyHostToDevice, s));
-
-    const auto t0 = std::chrono::steady_clock::now();
-    // r = A x
-    launch_fill(op->r.ptr, 0.0, N, s);
-    op->apply64(x, op->r.ptr, s);
-    launch_copy_listed<double>(op->r.ptr, x, op->cons.ptr, op->n_cons, s);
-    cg_init(op->st.ptr, b, op->r.ptr, inv_diag, N, op->part.ptr, abs_tol, rel_tol, max_steps, s);
-    const int *done = &op->st.ptr->done;
-    int batch = 4;
-    for (int it = 0;;) {
-      for (int k = 0; k < batch; ++k, ++it) {
-        cg_direction(op->st.ptr, op->r.ptr, inv_diag, op->pdir.ptr, op->ap.ptr, x, N, s);
-        op->apply64(op->pdir.ptr, op->ap.ptr, s, op->part.ptr, done);
-        cg_constrained(op->st.ptr, op->cons.ptr, op->n_cons, op->pdir.ptr, op->ap.ptr,
-                       op->part.ptr + nb_apply, s);
-        cg_alpha(op->st.ptr, op->part.ptr, nb_apply + nb_cons, s);
-        cg_update(op->st.ptr, op->r.ptr, op->ap.ptr, inv_diag, N, op->part.ptr, s);
-        cg_check(op->st.ptr, op->part.ptr, nb_vec, s);
-      }
-      MFX_HIP(hipGetLastError());
-      MFX_HIP(hipMemcpyAsync(op->h_done, done, sizeof(int), hipMemcpyDeviceToHost, s));
-      MFX_HIP(hipStreamSynchronize(s));
-      if (*op->h_done) break;
-      if (it > max_steps + 8) break;  // device enforces max_steps; this bounds the host loop
-      batch = std::min(batch * 2, 64);
-    }
-    cg_finish(op->st.ptr, x, op->pdir.ptr, N, s);
-    CgState fin{};
-    MFX_HIP(hipMemcpyAsync(&fin, op->st.ptr, sizeof(CgState), hipMemcpyDeviceToHost, s));
-    MFX_HIP(hipStreamSynchronize(s));
-    const auto t1 = std::chrono::steady_clock::now();
-    res->steps = fin.steps;
-    res->converged = fin.done == 1;
-    res->initial_value = fin.initial;
-    res->last_value = fin.last;
-    res->seconds = std::chrono::duration<double>(t1 - t0).count();
-    if (fin.done != 1)
-      throw Error{fin.breakdown ? MFX_ERR_BREAKDOWN : MFX_ERR_NOT_CONVERGED,
-                  "CG did not converge: steps=" + std::to_string(fin.steps) +
-                      " residual=" + std::to_string(fin.last)};
+    const CgWork w{op->r.ptr, op->pdir.ptr, op->ap.ptr, op->part.ptr, op->st.ptr, op->h_done};
+    cg_run(*op, x, b, inv_diag, max_steps, abs_tol, rel_tol, w, res, as_stream(stream));
   });
 }
 

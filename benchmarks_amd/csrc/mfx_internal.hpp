@@ -40,6 +40,14 @@ void launch_apply(int p, int nq, bool collocation, const T *B, const T *D,
                   const T *G, const T *src, T *dst, const uint32_t *map,
                   uint32_t n_cells, hipStream_t s, double *dot_partials = nullptr,
                   const int *skip = nullptr);
+// Non-collocated (Gauss) apply with the metric terms of the trilinear cells
+// computed per quadrature point from their vertices (cellX [E][8][3]), xq and
+// wq the 1D quadrature points and weights on [0,1].
+template <typename T>
+void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *xq, const T *wq,
+                      const double *cellX, const T *src, T *dst, const uint32_t *map,
+                      uint32_t n_cells, hipStream_t s, double *dot_partials = nullptr,
+                      const int *skip = nullptr);
 template <typename T>
 void launch_evec(int p, int nq, bool collocation, const T *B, const T *D,
                  const T *G, const T *in, T *out, uint32_t n_cells,

@@ -152,6 +152,9 @@ struct mfx_op {
                const int *skip = nullptr) const {
     if (patch)
       launch_patch<double>(patch_op64(), src, dst, s, partials, skip);
+    else if (geometry == MFX_GEOM_ON_THE_FLY)
+      launch_apply_otf<double>(p, nq, B.ptr, D.ptr, xq.ptr, wq.ptr, cellX.ptr, src, dst, map.ptr,
+                               E, s, partials, skip);
     else
       launch_apply<double>(p, nq, colloc, B.ptr, D.ptr, G.ptr, src, dst, map.ptr, E, s, partials,
                            skip);
@@ -159,8 +162,27 @@ struct mfx_op {
   void apply32(const float *src, float *dst, hipStream_t s) const {
     if (patch)
       launch_patch<float>(patch_op32(), src, dst, s);
+    else if (geometry == MFX_GEOM_ON_THE_FLY)
+      launch_apply_otf<float>(p, nq, B32.ptr, D32.ptr, xq32.ptr, wq32.ptr, cellX.ptr, src, dst,
+                              map.ptr, E, s);
     else
       launch_apply<float>(p, nq, colloc, B32.ptr, D32.ptr, G32.ptr, src, dst, map.ptr, E, s);
   }
 };
 
+namespace mfx {
+
+// Work buffers of one CG solve: three N-vectors, cg_partial_count(op)
+// partial sums, the device state, and a host word for the `done` flag
+// (pinned, or nullptr to read it into pageable host memory).
+struct CgWork {
+  double *r, *p, *ap, *part;
+  CgState *st;
+  int *h_done;
+};
+size_t cg_partial_count(const mfx_op &op);
+// The CG iteration of mfx_cg_solve on the given work buffers (capi.cpp).
+void cg_run(mfx_op &op, double *x, const double *b, const double *inv_diag, int max_steps,
+            double abs_tol, double rel_tol, const CgWork &w, mfx_cg_result *res, hipStream_t s);
+
+}  // namespace mfx

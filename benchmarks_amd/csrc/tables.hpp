@@ -14,6 +14,21 @@
 
 namespace mfx {
 
+// 1/x from the hardware reciprocal estimate refined by Newton steps
+// (e = 1 - x r; r += r e), two steps in FP64, one in FP32.
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ float recip(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, r, 1.0f);
+  return __builtin_fmaf(r, e, r);
+}
+
 constexpr double cx_cos(double t) {  // Taylor series, |t| <= pi
   double s = 1.0, term = 1.0;
   for (int k = 1; k < 40; ++k) {

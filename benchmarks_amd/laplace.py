@@ -96,10 +96,11 @@ class LaplaceOperator:
 
     @classmethod
     def from_arrays(cls, degree, n_q_points_1d, quadrature, map_, cell_vertices, constrained,
-                    n_dofs, with_fp32=False):
+                    n_dofs, with_fp32=False, geometry: int = 0):
         """General (unstructured) creation from host numpy arrays: the DoF map
         [n_cells][(p+1)^3] (0xFFFFFFFF = constrained), the 8 vertices of each
-        cell and the list of constrained DoFs."""
+        cell and the list of constrained DoFs; geometry 1 computes the metric
+        terms from the vertices (Gauss quadrature only)."""
         import numpy as np
         m = np.ascontiguousarray(map_, dtype=np.uint32)
         cv = np.ascontiguousarray(cell_vertices, dtype=np.float64)
@@ -107,7 +108,7 @@ class LaplaceOperator:
         nm3 = (degree + 1) ** 3
         d = _lib.OpDesc()
         d.degree, d.n_q_points_1d, d.quadrature = degree, n_q_points_1d, quadrature
-        d.geometry = _lib.GEOM_STORED
+        d.geometry = geometry
         d.n_cells = m.size // nm3
         d.n_dofs = n_dofs
         d.map = m.ctypes.data

@@ -52,6 +52,19 @@ def test_bp3_cg_iterations_match_reference_logs(gpu, cycle, nq):
     assert ctl.last_value() <= 1e-9 * ctl.initial_value()
 
 
+@pytest.mark.parametrize("cycle", [8, 9, 10, 11, 12, 20])
+def test_bp3_cg_iterations_with_on_the_fly_geometry(gpu, cycle):
+    """The same known answers with the Gauss kernel computing the metric
+    terms from the cell vertices (no stored G)."""
+    from benchmarks_amd.laplace import LaplaceOperator
+    n, lo, hi = cycle_mesh(cycle)
+    print(f"cycle {cycle} n={n} on the fly", flush=True)
+    op = LaplaceOperator(4, 6, O.GAUSS, n_cells=n, lo=lo, hi=hi, geometry=1)
+    assert op.m() == KAT2_DOFS[cycle] and op.info.G is None
+    x, ctl = _solve(op, op.rhs(0))
+    assert ctl.last_step() == KAT1[cycle]
+
+
 @pytest.mark.parametrize("cycle", [8, 9, 10])
 def test_bp3_cg_solution_matches_oracle(gpu, cycle):
     from benchmarks_amd.laplace import LaplaceOperator
