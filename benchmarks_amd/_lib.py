@@ -106,6 +106,9 @@ _SIGS = {
     "mfx_op_dof_coordinates": (C.c_int, [_P, _P, _P]),
     "mfx_cg_solve": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double,
                                C.POINTER(CgResult), _P]),
+    "mfx_cg_solve_noalloc": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_double, C.c_double,
+                                       C.POINTER(CgResult), _P]),
+    "mfx_cg_noalloc_max_dofs": (C.c_uint64, []),
     "mfx_vec_random": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P]),
     "mfx_vec_sine_modes": (C.c_int, [_P, C.c_uint64, C.c_int, _P, _P, _P, _P]),
     "mfx_vec_zero_constrained": (C.c_int, [_P, _P, _P]),

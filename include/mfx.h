@@ -232,6 +232,20 @@ mfx_status mfx_cg_solve(mfx_op *op, double *x, const double *b,
                         const double *inv_diag, int max_steps, double abs_tol,
                         double rel_tol, mfx_cg_result *result, void *stream);
 
+/* The same solve without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device or pinned host memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The work vectors, partial sums and solver state live in a fixed device
+ * array of the library (one per GPU, held until the process ends), so the
+ * operator may have at most mfx_cg_noalloc_max_dofs() DoFs (2,097,152);
+ * larger operators are rejected with MFX_ERR_INVALID_ARG before any work.
+ * Calls that overlap on one GPU share that array and race.  Synchronous;
+ * same arguments, stopping rule and results as mfx_cg_solve. */
+mfx_status mfx_cg_solve_noalloc(mfx_op *op, double *x, const double *b,
+                                const double *inv_diag, int max_steps, double abs_tol,
+                                double rel_tol, mfx_cg_result *result, void *stream);
+uint64_t mfx_cg_noalloc_max_dofs(void);
+
 /* ----------------------------------------------------------------------
  * Vector helpers (deal.II LA::distributed::Vector device operations).
  * ---------------------------------------------------------------------- */
