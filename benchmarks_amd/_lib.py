@@ -97,6 +97,7 @@ _SIGS = {
     "mfx_op_n_dofs": (C.c_uint64, [_P]),
     "mfx_op_n_cells": (C.c_uint32, [_P]),
     "mfx_op_get_info": (C.c_int, [_P, C.POINTER(OpInfo)]),
+    "mfx_op_set_deterministic": (C.c_int, [_P, C.c_int]),
     "mfx_op_vmult": (C.c_int, [_P, _P, _P, _P]),
     "mfx_op_vmult_add": (C.c_int, [_P, _P, _P, _P]),
     "mfx_op_vmult_f32": (C.c_int, [_P, _P, _P, _P]),

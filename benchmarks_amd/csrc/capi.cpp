@@ -377,6 +377,27 @@ mfx_status mfx_op_get_info(const mfx_op *op, mfx_op_info *info) {
   });
 }
 
+mfx_status mfx_op_set_deterministic(mfx_op *op, int on) {
+  return guard([&] {
+    MFX_REQUIRE(op, "null pointer argument");
+    MFX_REQUIRE(!on || op->patch,
+                "deterministic scatter is implemented for the patch kernel (box operators with "
+                "GLL collocation and on-the-fly geometry)");
+    MFX_HIP(hipDeviceSynchronize());  // no apply of this operator in flight
+    if (on) {
+      const PatchShape sh = op->shape;
+      const uint64_t n = (uint64_t)patch_count(op->lay) * (sh.bx * op->p + 1) *
+                         (sh.by * op->p + 1) * (sh.bz * op->p + 1);
+      if (op->pbuf64.n != n) op->pbuf64.alloc(n);
+      if (op->D32.ptr && op->pbuf32.n != n) op->pbuf32.alloc(n);
+    } else {
+      op->pbuf64.reset();
+      op->pbuf32.reset();
+    }
+    op->det = on != 0;
+  });
+}
+
 mfx_status mfx_op_vmult(mfx_op *op, double *dst, const double *src, void *stream) {
   return guard([&] {
     MFX_REQUIRE(op && dst && src, "null pointer argument");

@@ -110,6 +110,10 @@ struct mfx_op {
   PatchShape shape{};
   PatchGrid grid64{}, grid32{};
   DBuf<uint32_t> pmap;
+  // deterministic scatter (mfx_op_set_deterministic): per-patch sums
+  bool det = false;
+  DBuf<double> pbuf64;
+  DBuf<float> pbuf32;
   // CG scratch, allocated on first use
   DBuf<double> r, pdir, ap, part;
   DBuf<CgState> st;
@@ -129,6 +133,7 @@ struct mfx_op {
     o.xq = xq.ptr;
     o.cellX = cellX.ptr;
     o.pmap = pmap.ptr;
+    o.pbuf = det ? pbuf64.ptr : nullptr;
     return o;
   }
   PatchOp<float> patch_op32() const {
@@ -142,10 +147,11 @@ struct mfx_op {
     o.xq = xq32.ptr;
     o.cellX = cellX.ptr;
     o.pmap = pmap.ptr;
+    o.pbuf = det ? pbuf32.ptr : nullptr;
     return o;
   }
   uint32_t apply_blocks() const {
-    return patch ? grid64.dot : apply_n_blocks(p, nq, colloc, E);
+    return patch ? (det ? grid64.det_dot : grid64.dot) : apply_n_blocks(p, nq, colloc, E);
   }
   // dst += A_free src (no zeroing, no constrained rows)
   void apply64(const double *src, double *dst, hipStream_t s, double *partials = nullptr,

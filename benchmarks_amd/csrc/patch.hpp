@@ -18,9 +18,10 @@ struct PatchLayout {
 
 // Workgroups of the persistent patch kernel: as many as the device holds at
 // once (resident workgroups per CU x CUs), at most one per patch.  `dot` is
-// the grid of the variant that also returns per-workgroup p.Ap partials.
+// the grid of the variant that also returns per-workgroup p.Ap partials; the
+// det_ grids are those of the deterministic-scatter variants.
 struct PatchGrid {
-  uint32_t plain = 0, dot = 0;
+  uint32_t plain = 0, dot = 0, det_plain = 0, det_dot = 0;
 };
 
 // Device pointers and parameters one patch-kernel launch needs.
@@ -35,6 +36,9 @@ struct PatchOp {
   const T *xq = nullptr;         // 1D quadrature points [nq]
   const double *cellX = nullptr; // cell vertices [n_cells][8][3], lexicographic cells
   const uint32_t *pmap = nullptr;
+  // deterministic scatter: per-patch sums [n_patches][P3], added per node in
+  // patch order by a second kernel (nullptr: atomic scatter)
+  T *pbuf = nullptr;
 };
 
 // Patches of many box operators of one degree in one launch (batched

@@ -164,6 +164,11 @@ class LaplaceOperator:
     def Tvmult(self, dst, src) -> None:
         self.vmult(dst, src)
 
+    def set_deterministic(self, on: bool = True) -> None:
+        """Bitwise-reproducible applies: per-patch sums added per DoF in a
+        fixed order instead of atomically (patch-kernel operators only)."""
+        check(lib().mfx_op_set_deterministic(self._h, int(on)))
+
     def _check_vec(self, v: torch.Tensor):
         if v.numel() != self.m():
             raise ValueError(f"vector size {v.numel()} != m() = {self.m()}")

@@ -25,8 +25,9 @@ Given the fine operator, half of the pairs are instead what the cycle hands
 the network (restricted_fields): a smooth fine field, the cycle's own FP32
 Chebyshev pre-smoothing and residual, restricted to the coarse grid by the
 library (mfx_nn_restricted_residual), and the exact coarse solution.  The
-mix took 31-32 flexible-CG steps on the measured problem (32 pairs) against
-36-37 for coarse fields alone and 39-51 for restricted fields alone.
+mix took 31-32 flexible-CG steps on the measured problem with 32 pairs (30
+with 4096) against 36-37 for coarse fields alone and 39-51 for restricted
+fields alone.
 """
 from __future__ import annotations
 

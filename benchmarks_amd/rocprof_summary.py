@@ -114,9 +114,22 @@ def counters(pmc_csvs, pattern, points=None, title="SQ counters"):
     if "SQ_LDS_BANK_CONFLICT" in avg and "SQ_LDS_IDX_ACTIVE" in avg:
         out.append(f"| LDS bank-conflict share of LDS-active cycles | "
                    f"{avg['SQ_LDS_BANK_CONFLICT'] / avg['SQ_LDS_IDX_ACTIVE']:.3f} |")
-    if "SQ_ACTIVE_INST_VALU" in avg and "GRBM_GUI_ACTIVE" in avg:
-        out.append(f"| VALU busy (SQ_ACTIVE_INST_VALU / CU / GRBM_GUI_ACTIVE) | "
-                   f"{avg['SQ_ACTIVE_INST_VALU'] / 256 / avg['GRBM_GUI_ACTIVE']:.3f} |")
+    if "GRBM_GUI_ACTIVE" in avg:
+        # GRBM_GUI_ACTIVE comes summed over the 8 XCDs (8x the kernel's cycles)
+        cyc = avg["GRBM_GUI_ACTIVE"] / 8.0
+        out.append(f"| kernel cycles (GRBM_GUI_ACTIVE / 8 XCDs) | {cyc:.4g} |")
+        if "SQ_ACTIVE_INST_VALU" in avg:
+            out.append(f"| VALU busy (4 SQ_ACTIVE_INST_VALU / 1024 SIMDs / cycles) | "
+                       f"{4.0 * avg['SQ_ACTIVE_INST_VALU'] / 1024 / cyc:.3f} |")
+        if "SQ_LDS_IDX_ACTIVE" in avg:
+            out.append(f"| LDS busy (SQ_LDS_IDX_ACTIVE / 256 CUs / cycles) | "
+                       f"{avg['SQ_LDS_IDX_ACTIVE'] / 256 / cyc:.3f} |")
+        if "SQ_LDS_BANK_CONFLICT" in avg:
+            out.append(f"| LDS bank-conflict share of kernel cycles | "
+                       f"{avg['SQ_LDS_BANK_CONFLICT'] / 256 / cyc:.3f} |")
+    if "SQ_WAIT_INST_LDS" in avg and avg.get("SQ_WAVE_CYCLES"):
+        out.append(f"| wave-cycles waiting to issue LDS (SQ_WAIT_INST_LDS / SQ_WAVE_CYCLES) | "
+                   f"{avg['SQ_WAIT_INST_LDS'] / avg['SQ_WAVE_CYCLES']:.3f} |")
     if "SQ_INSTS_LDS" in avg and points:
         out.append(f"| LDS instructions per quadrature point (x 64 lanes) | "
                    f"{64.0 * avg['SQ_INSTS_LDS'] / float(points):.2f} |")

@@ -197,6 +197,16 @@ mfx_status mfx_op_vmult_add(mfx_op *op, double *dst, const double *src,
 mfx_status mfx_op_vmult_f32(mfx_op *op, float *dst, const float *src,
                             void *stream);
 
+/* Deterministic scatter (on != 0): instead of adding its sums to dst with
+ * atomics, each patch of cells stores them, and a second pass adds the
+ * (up to 8) patch sums of every DoF in a fixed order, so applies -- and the
+ * CG iterations on them -- are bitwise reproducible from run to run.  The
+ * reference's scatter is atomic (CEED_bp/include/bk3_kokkos_kernel.h:356-382)
+ * and not reproducible.  Operators on the patch kernel only (box meshes,
+ * GLL collocation, on-the-fly geometry); holds one value per patch DoF
+ * (FP64, and FP32 with with_fp32) while on.  Synchronizes the device. */
+mfx_status mfx_op_set_deterministic(mfx_op *op, int on);
+
 /* Diagonal of the assembled operator (constrained rows: 1), the quantity
  * MatrixFreeTools::compute_diagonal fills at bp5_kokkos/benchmark.cc:231-236;
  * and its inverse per bp5_kokkos/benchmark.cc:240-250 (1/d if d > 0, else 1). */

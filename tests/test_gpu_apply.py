@@ -87,6 +87,55 @@ def test_vmult_on_the_fly_geometry_matches_oracle(gpu, p, n):
     assert np.array_equal(yg[cons], x[cons])
 
 
+@pytest.mark.parametrize("p", range(1, 9))
+@pytest.mark.parametrize("n", [(3, 4, 5), (1, 1, 1)])
+def test_deterministic_scatter_matches_oracle_bit_for_bit_on_repeat(gpu, p, n):
+    """Deterministic scatter (mfx_op_set_deterministic): repeated applies are
+    bitwise identical (FP64 and FP32); they agree with the oracle to 1e-12
+    and with the atomic scatter to 1e-13 (only the summation order differs);
+    vmult_add adds to dst; switching the mode off restores the atomic form.
+    Meshes whose cell counts are not multiples of the patch size leave
+    partial patches, whose nodes the gather must still find."""
+    from benchmarks_amd.laplace import LaplaceOperator
+    print(f"case p={p} n={n} deterministic", flush=True)
+    op = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05, geometry=1, with_fp32=True)
+    P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
+    x = torch.from_numpy(O.splitmix64_uniform(1, P.N)).cuda()
+    ya = op.initialize_dof_vector()
+    op.vmult(ya, x)
+    op.set_deterministic(True)
+    y1, y2 = op.initialize_dof_vector(), op.initialize_dof_vector()
+    op.vmult(y1, x)
+    op.vmult(y2, x)
+    assert torch.equal(y1, y2)
+    ref = P.vmult(x.cpu().numpy())
+    assert _rel(y1.cpu().numpy(), ref) <= TOL64
+    assert (torch.linalg.norm(y1 - ya) / torch.linalg.norm(ya)).item() <= 1e-13
+    z = torch.ones_like(x)
+    op.vmult_add(z, x)
+    cons = torch.from_numpy(P.constrained.astype(bool)).cuda()
+    assert torch.equal(z[cons], torch.ones_like(z[cons]))
+    if (~cons).any():  # a single cell has no unconstrained DoF
+        assert ((z - 1.0 - y1)[~cons].abs().max() / y1.abs().max()).item() <= 1e-15
+    f1, f2 = (torch.zeros(op.m(), dtype=torch.float32, device="cuda") for _ in range(2))
+    op.vmult(f1, x.float())
+    op.vmult(f2, x.float())
+    assert torch.equal(f1, f2)
+    assert _rel(f1.double().cpu().numpy(), ref) <= 1e-5
+    op.set_deterministic(False)
+    y3 = op.initialize_dof_vector()
+    op.vmult(y3, x)
+    assert (torch.linalg.norm(y3 - y1) / torch.linalg.norm(y1)).item() <= 1e-13
+
+
+def test_deterministic_scatter_needs_the_patch_kernel(gpu):
+    from benchmarks_amd._lib import MfxError
+    from benchmarks_amd.laplace import LaplaceOperator
+    op = LaplaceOperator(3, 5, O.GAUSS, n_cells=(2, 2, 2))
+    with pytest.raises(MfxError):
+        op.set_deterministic(True)
+
+
 @pytest.mark.parametrize("p,nq", [(p, p + 2) for p in range(1, 9)] +
                          [(p, p + 1) for p in (1, 2, 4, 6, 8)])
 def test_gauss_on_the_fly_geometry_matches_oracle(gpu, p, nq):

@@ -143,6 +143,30 @@ def test_cg_max_steps_raises(gpu):
     assert ei.value.last_step == 5
 
 
+@pytest.mark.parametrize("p,n", [(8, 3), (4, 6)])
+def test_deterministic_scatter_makes_cg_reproducible(gpu, p, n):
+    """With the deterministic scatter every reduction and every apply has a
+    fixed order: three Jacobi-CG solves take the same steps and return
+    bitwise-identical iterates (p = 8 on 3^3 cells is the knife-edge case of
+    the atomic scatter), with the oracle's step count (knife-edge rule)."""
+    from benchmarks_amd.laplace import LaplaceOperator
+    op = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05, geometry=1)
+    op.set_deterministic(True)
+    P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
+    b = O.splitmix64_uniform(1, P.N)
+    b[P.constrained.astype(bool)] = 0.0
+    bt = torch.as_tensor(b, device="cuda")
+    inv = op.compute_inverse_diagonal()
+    runs = [_solve(op, bt, inv, reduction=1e-8, tolerance=1e-300, max_steps=100000)
+            for _ in range(3)]
+    for x, ctl in runs[1:]:
+        assert ctl.last_step() == runs[0][1].last_step()
+        assert torch.equal(x, runs[0][0])
+    xo, r, hist = P.cg(b, inv_diag=P.inverse_diagonal(), rel_tol=1e-8, abs_tol=1e-300,
+                       max_steps=100000)
+    assert_same_steps(runs[0][1].last_step(), r.steps, hist, 1e-8)
+
+
 def test_cg_max_steps_zero_takes_no_step(gpu):
     """max_steps = 0 on an unconverged start: NoConvergence at step 0 with x
     unchanged (deal.II's SolverControl::check at step 0), as the oracle."""
