@@ -57,7 +57,7 @@ class CgResult(C.Structure):
 class MgDesc(C.Structure):
     _fields_ = [("fine", BoxDesc), ("smoother_degree", C.c_int), ("smoothing_range", C.c_double),
                 ("eig_cg_n_iterations", C.c_int), ("eig_safety", C.c_double),
-                ("coarse_max_dofs", C.c_uint32)]
+                ("coarse_max_dofs", C.c_uint32), ("fine_smoother_n_q_points_1d", C.c_int)]
 
 
 class MgLevelInfo(C.Structure):

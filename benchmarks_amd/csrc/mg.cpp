@@ -11,7 +11,9 @@
 //   smoother Chebyshev iteration with the Jacobi preconditioner, eigenvalue
 //            interval [lambda_max / range, lambda_max] with lambda_max from a
 //            few CG (Lanczos) steps times a safety factor (the semantics of
-//            deal.II's PreconditionChebyshev);
+//            deal.II's PreconditionChebyshev); on request the fine level
+//            smooths with its own operator of fewer Gauss points (same
+//            mesh, degree and DoFs), while the outer CG keeps the fine one;
 //   transfer tensor products of 1D nodal interpolation matrices (mg_kernels.hip);
 //   coarse   FP64 explicit inverse of the coarsest operator on its free DoFs;
 //   cycle    V-cycle in FP32 (operators, smoother, transfers), zero initial
@@ -229,10 +231,13 @@ using namespace mfx;
 struct mfx_mg {
   struct Level {
     OpPtr op;
+    OpPtr sop;  // smoothing operator when it differs from op (fine level only)
     uint32_t dims[3] = {0, 0, 0};
     DBuf<float> dinv, x, b, t, d;
     double lmax = 0.0, lmin = 0.0;
     Transfer to_coarse;  // empty on the coarsest level
+    // operator of the level's smoothing and residual (FP32)
+    mfx_op &sm() const { return sop ? *sop : *op; }
   };
   std::vector<Level> lv;
   int degree = 5;
@@ -244,7 +249,7 @@ struct mfx_mg {
   DBuf<double> r, z, p, ap, part;
 
   void smooth(Level &L, bool zero_initial, hipStream_t s) {
-    chebyshev(*L.op, L.x.ptr, L.b.ptr, L.dinv.ptr, L.t.ptr, L.d.ptr, degree, L.lmin, L.lmax,
+    chebyshev(L.sm(), L.x.ptr, L.b.ptr, L.dinv.ptr, L.t.ptr, L.d.ptr, degree, L.lmin, L.lmax,
               zero_initial, s);
   }
 
@@ -258,7 +263,7 @@ struct mfx_mg {
     }
     Level &Cl = lv[l + 1];
     smooth(L, true, s);
-    vmult32(*L.op, L.t.ptr, L.x.ptr, s);
+    vmult32(L.sm(), L.t.ptr, L.x.ptr, s);
     launch_residual_f32(L.t.ptr, L.b.ptr, L.t.ptr, L.op->N, s);
     restrict_<float>(L.to_coarse.R32, L.dims, Cl.dims, L.t.ptr, L.d.ptr, L.t.ptr, Cl.b.ptr, s);
     launch_set_listed_f32(Cl.b.ptr, 0.0f, Cl.op->cons.ptr, Cl.op->n_cons, s);
@@ -351,8 +356,19 @@ mfx_status mfx_mg_create(const mfx_mg_desc *d, mfx_mg **out) {
     // halved meshes while the free DoFs exceed the direct-solve limit
     std::vector<mfx_box_desc> descs;
     mfx_box_desc cur = d->fine;
-    cur.with_fp32 = 1;
+    const int q_smooth = d->fine_smoother_n_q_points_1d;
+    const bool own_smoother = q_smooth > 0 && q_smooth != cur.n_q_points_1d;
+    MFX_REQUIRE(q_smooth >= 0, "fine_smoother_n_q_points_1d must be >= 0");
+    MFX_REQUIRE(!own_smoother || (cur.quadrature == MFX_QUAD_GAUSS && q_smooth > cur.degree),
+                "a separate fine smoothing operator needs Gauss quadrature with q > p");
+    // the fine operator serves the outer FP64 CG; it smooths in FP32 unless
+    // a separate smoothing operator is asked for
+    mfx_box_desc smooth_desc = cur;
+    smooth_desc.n_q_points_1d = q_smooth;
+    smooth_desc.with_fp32 = 1;
+    cur.with_fp32 = own_smoother ? 0 : 1;
     descs.push_back(cur);
+    cur.with_fp32 = 1;
     // levels below the fine one: Gauss q = p+1 under a Gauss fine operator
     // (exact for the stiffness of affine cells; the fine level keeps its own
     // rule), otherwise the fine operator's q - p
@@ -376,6 +392,7 @@ mfx_status mfx_mg_create(const mfx_mg_desc *d, mfx_mg **out) {
     for (const auto &ld : descs) {
       mfx_mg::Level L;
       L.op.reset(make_level_op(ld));
+      if (own_smoother && mg->lv.empty()) L.sop.reset(make_level_op(smooth_desc));
       for (int k = 0; k < 3; ++k) L.dims[k] = ld.n_cells[k] * ld.degree + 1;
       mg->lv.push_back(std::move(L));
     }
@@ -385,7 +402,7 @@ mfx_status mfx_mg_create(const mfx_mg_desc *d, mfx_mg **out) {
       const uint64_t N = L.op->N;
       DBuf<double> inv;
       inv.alloc(N);
-      const mfx_status st = mfx_op_inverse_diagonal(L.op.get(), inv.ptr, s);
+      const mfx_status st = mfx_op_inverse_diagonal(&L.sm(), inv.ptr, s);
       if (st != MFX_OK) throw Error{st, mfx_last_error()};
       L.dinv.alloc(N);
       launch_to_float(inv.ptr, L.dinv.ptr, N, s);
@@ -394,7 +411,7 @@ mfx_status mfx_mg_create(const mfx_mg_desc *d, mfx_mg **out) {
       L.t.alloc(N);
       L.d.alloc(N);
       if (&L != &mg->lv.back()) {
-        L.lmax = safety * estimate_lambda_max(*L.op, inv.ptr, eig_its, s);
+        L.lmax = safety * estimate_lambda_max(L.sm(), inv.ptr, eig_its, s);
         L.lmin = L.lmax / range;
       }
       MFX_HIP(hipStreamSynchronize(s));

@@ -265,7 +265,7 @@ class PreconditionMG:
                  n_cells=(8, 8, 8), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform: float = 0.0,
                  geometry: int = 0, smoother_degree: int = 5, smoothing_range: float = 15.0,
                  eig_cg_n_iterations: int = 12, eig_safety: float = 1.2,
-                 coarse_max_dofs: int = 2000):
+                 coarse_max_dofs: int = 2000, fine_smoother_n_q_points_1d: int = 0):
         if isinstance(n_cells, int):
             n_cells = (n_cells,) * 3
         d = _lib.MgDesc()
@@ -279,6 +279,7 @@ class PreconditionMG:
         d.smoother_degree, d.smoothing_range = smoother_degree, smoothing_range
         d.eig_cg_n_iterations, d.eig_safety = eig_cg_n_iterations, eig_safety
         d.coarse_max_dofs = coarse_max_dofs
+        d.fine_smoother_n_q_points_1d = fine_smoother_n_q_points_1d
         self._h = None
         h = C.c_void_p()
         check(lib().mfx_mg_create(C.byref(d), C.byref(h)))

@@ -298,6 +298,12 @@ typedef struct {
   int eig_cg_n_iterations;  /* CG (Lanczos) steps of the lambda_max estimate; 0: 12 */
   double eig_safety;        /* lambda_max = safety x estimate; 0: 1.2 */
   uint32_t coarse_max_dofs; /* largest free-DoF count solved directly; 0: 2000 */
+  /* Gauss points per direction of the fine level's FP32 smoothing operator
+   * (Chebyshev and the residual restricted to the next level); 0: the fine
+   * operator's own rule.  Under a Gauss q = p+2 fine operator, q = p+1
+   * integrates the stiffness of affine cells exactly at 58% of the points;
+   * the outer FP64 CG always applies the fine operator itself. */
+  int fine_smoother_n_q_points_1d;
 } mfx_mg_desc;
 
 typedef struct {

@@ -210,6 +210,7 @@ struct MGAdditionalData {
   unsigned eig_cg_n_iterations = 12;    // CG (Lanczos) steps of the estimate
   double eig_safety = 1.2;              // lambda_max = safety x estimate
   std::uint32_t coarse_max_dofs = 2000; // largest directly solved level
+  unsigned fine_smoother_n_q_points_1d = 0;  // fine smoothing rule (0: the fine operator's)
 };
 
 class PreconditionMG {
@@ -234,6 +235,7 @@ class PreconditionMG {
     d.eig_cg_n_iterations = static_cast<int>(data.eig_cg_n_iterations);
     d.eig_safety = data.eig_safety;
     d.coarse_max_dofs = data.coarse_max_dofs;
+    d.fine_smoother_n_q_points_1d = static_cast<int>(data.fine_smoother_n_q_points_1d);
     check(mfx_mg_create(&d, &mg_));
     mfx_mg_level_info info{};
     check(mfx_mg_get_level(mg_, 0, &info));

@@ -142,17 +142,25 @@ class Multigrid:
 
     def __init__(self, p, nq, quad, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform=0.0,
                  smoother_degree=5, smoothing_range=15.0, eig_iterations=12, eig_safety=1.2,
-                 coarse_max_dofs=2000, eig_seed=0x5EED):
+                 coarse_max_dofs=2000, eig_seed=0x5EED, fine_smoother_nq=None):
+        """fine_smoother_nq: Gauss points of the fine level's smoothing
+        operator (Chebyshev, and the residual handed to the next level) when
+        it differs from the fine operator's; the outer PCG keeps the fine
+        operator."""
         self.degree = smoother_degree
         self.levels = []
         for (dp, dnq, dq, dn) in level_descriptors(p, nq, quad, n, coarse_max_dofs):
             prob = O.make_problem(dp, dnq, dq, dn, lo, hi, deform)
-            self.levels.append({"problem": prob, "dinv": prob.inverse_diagonal()})
+            smooth = prob
+            if not self.levels and fine_smoother_nq and fine_smoother_nq != nq:
+                smooth = O.make_problem(dp, fine_smoother_nq, dq, dn, lo, hi, deform)
+            self.levels.append({"problem": prob, "smooth": smooth,
+                                "dinv": smooth.inverse_diagonal()})
         for l in range(len(self.levels) - 1):
             self.levels[l]["P"] = prolongation_3d(self.levels[l]["problem"],
                                                   self.levels[l + 1]["problem"])
         for L in self.levels[:-1]:
-            prob = L["problem"]
+            prob = L["smooth"]
             r0 = O.splitmix64_uniform(eig_seed, prob.N)
             r0[prob.constrained.astype(bool)] = 0.0
             L["lmax"] = eig_safety * lanczos_lambda_max(prob.vmult, L["dinv"], r0, eig_iterations)
@@ -170,7 +178,7 @@ class Multigrid:
 
     def vcycle(self, b, l=0):
         L = self.levels[l]
-        prob = L["problem"]
+        prob = L["smooth"]
         if l == len(self.levels) - 1:
             x = np.zeros(prob.N)
             x[self.free] = self.Ainv @ b[self.free]

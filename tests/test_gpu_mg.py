@@ -63,11 +63,12 @@ def test_transfer_rejects_unrelated_operators(gpu):
         transfer_prolongate(Af, Ad, _t(np.zeros(Ad.m())))
 
 
-def _pair(p, nq, quad, n, coarse_max, geometry=0):
+def _pair(p, nq, quad, n, coarse_max, geometry=0, smoother_q=0):
     from benchmarks_amd.laplace import PreconditionMG
     mg = PreconditionMG(p, nq, quad, n, deform=0.05, geometry=geometry,
-                        coarse_max_dofs=coarse_max)
-    om = M.Multigrid(p, nq, quad, n, deform=0.05, coarse_max_dofs=coarse_max)
+                        coarse_max_dofs=coarse_max, fine_smoother_n_q_points_1d=smoother_q)
+    om = M.Multigrid(p, nq, quad, n, deform=0.05, coarse_max_dofs=coarse_max,
+                     fine_smoother_nq=smoother_q or None)
     return mg, om
 
 
@@ -138,6 +139,48 @@ def test_mg_solve_matches_oracle(gpu):
     assert _rel(xg, xd) <= 1e-7
     # FP64 certification of the returned solution by the oracle's operator
     assert np.linalg.norm(b - prob.vmult(xg)) <= 2e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("geometry", [0, 1])
+def test_fine_smoother_quadrature_matches_oracle(gpu, geometry):
+    """Fine level smoothing with its own Gauss q = p+1 operator under a
+    q = p+2 fine operator: Chebyshev interval, V-cycle and MG-CG steps as the
+    oracle's; the solution is that of the q = p+2 operator (FP64-certified),
+    and the V-cycle differs from the one that smooths with the fine operator
+    (the option takes effect)."""
+    from benchmarks_amd.laplace import ReductionControl, SolverCG
+    print(f"case fine smoother q=5 geometry={geometry}", flush=True)
+    mg, om = _pair(4, 6, O.GAUSS, 4, 30, geometry, smoother_q=5)
+    g = mg.level(0)
+    assert (g["degree"], g["n_q_points_1d"]) == (4, 6)
+    assert g["lambda_max"] == pytest.approx(om.levels[0]["lmax"], rel=1e-6)
+    A = mg.operator()
+    r = O.splitmix64_uniform(8, A.m())
+    z = A.initialize_dof_vector()
+    mg.vmult(z, _t(r))
+    assert _rel(z.cpu().numpy(), om.vmult(r)) <= 1e-4
+    mg6, om6 = _pair(4, 6, O.GAUSS, 4, 30, geometry)
+    z6 = A.initialize_dof_vector()
+    mg6.vmult(z6, _t(r))
+    assert _rel(z.cpu().numpy(), z6.cpu().numpy()) >= 3e-4  # 9.5e-4 in the oracle
+    prob = om.levels[0]["problem"]
+    b = prob.rhs(1)
+    b[prob.constrained.astype(bool)] = 0
+    x = A.initialize_dof_vector()
+    ctl = ReductionControl(100, 1e-300, 1e-10)
+    SolverCG(ctl).solve(A, x, _t(b), mg)
+    _, steps, _ = om.pcg(b, rel_tol=1e-10)
+    assert abs(ctl.last_step() - steps) <= 1, (ctl.last_step(), steps)
+    assert np.linalg.norm(b - prob.vmult(x.cpu().numpy())) <= 2e-10 * np.linalg.norm(b)
+
+
+def test_fine_smoother_quadrature_is_checked(gpu):
+    from benchmarks_amd._lib import MfxError
+    from benchmarks_amd.laplace import PreconditionMG
+    with pytest.raises(MfxError):  # GLL collocation has no separate smoother rule
+        PreconditionMG(2, 3, O.GLL, 4, geometry=1, fine_smoother_n_q_points_1d=4)
+    with pytest.raises(MfxError):  # q must exceed p
+        PreconditionMG(4, 6, O.GAUSS, 4, coarse_max_dofs=30, fine_smoother_n_q_points_1d=4)
 
 
 def test_mg_beats_jacobi_cg(gpu):

@@ -145,6 +145,30 @@ def test_vcycle_contracts_h_independently():
     assert abs(rates[0] - rates[1]) < 0.1, rates
 
 
+def test_fine_smoother_rule_is_exact_on_affine_cells():
+    """Gauss q = p+1 integrates the stiffness of affine cells exactly, so a
+    fine level that smooths with q = p+1 under a q = p+2 operator gives the
+    same V-cycle to rounding on an undeformed mesh; on deformed cells it does
+    not, and PCG still converges to the q = p+2 operator's solution."""
+    r = O.splitmix64_uniform(5, M.Multigrid(4, 6, O.GAUSS, 4, coarse_max_dofs=30).levels[0]["problem"].N)
+    a = M.Multigrid(4, 6, O.GAUSS, 4, coarse_max_dofs=30)
+    b = M.Multigrid(4, 6, O.GAUSS, 4, coarse_max_dofs=30, fine_smoother_nq=5)
+    assert b.levels[0]["smooth"].nq == 5 and b.levels[0]["problem"].nq == 6
+    za, zb = a.vmult(r), b.vmult(r)
+    assert np.linalg.norm(za - zb) <= 1e-11 * np.linalg.norm(za)
+    a = M.Multigrid(4, 6, O.GAUSS, 4, deform=0.05, coarse_max_dofs=30)
+    b = M.Multigrid(4, 6, O.GAUSS, 4, deform=0.05, coarse_max_dofs=30, fine_smoother_nq=5)
+    za, zb = a.vmult(r), b.vmult(r)
+    assert np.linalg.norm(za - zb) >= 1e-4 * np.linalg.norm(za)
+    prob = b.levels[0]["problem"]
+    rhs = prob.rhs(1)
+    rhs[prob.constrained.astype(bool)] = 0
+    x, steps, hist = b.pcg(rhs, rel_tol=1e-10)
+    assert hist[-1] <= 1e-10 * hist[0] and steps <= 15
+    xj, _, _ = prob.cg(rhs, inv_diag=prob.inverse_diagonal(), rel_tol=1e-13, abs_tol=1e-300)
+    assert np.linalg.norm(x - xj) <= 1e-8 * np.linalg.norm(xj)
+
+
 def test_mg_pcg_solves():
     mg = M.Multigrid(4, 6, O.GAUSS, 4, deform=0.05, coarse_max_dofs=30)
     prob = mg.levels[0]["problem"]
