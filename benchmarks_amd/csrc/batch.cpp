@@ -52,6 +52,8 @@ struct mfx_batch {
     // patch coordinates), instance -> (cells per direction, first cell),
     // concatenated cell vertices
     PatchBatchOp patches;
+    DBuf<double> ppart;  // p.Ap partials of the patch kernel, [instance][workgroup][wave]
+    uint32_t waves = 0;
     DBuf<uint32_t> pmap;
     DBuf<uint4> meta, inst;
     DBuf<double> cellX, w, xq;
@@ -69,6 +71,10 @@ struct mfx_batch {
   DBuf<Seg> segs;
   uint32_t n_segs = 0;
   DBuf<uint32_t> inst_seg;
+  DBuf<uint8_t> cmask;  // 1 on constrained rows
+  // fused p.Ap: per instance its range of the patch kernel's partials
+  DBuf<PapRange> pap_ranges;
+  DBuf<double> part_c;  // constrained-row share of p.Ap per segment
   // CG scratch
   DBuf<BatchCg> st;
   DBuf<double> r, p, ap, t, part, part2, ones;
@@ -97,12 +103,25 @@ struct mfx_batch {
     apply(src, dst, nullptr, s);
     launch_copy_listed<double>(dst, src, cons.ptr, n_cons, s);
   }
+  // p.Ap comes from the patch kernel when every degree uses it
+  bool fused_pap() const {
+    for (auto &g : groups)
+      if (!g.on_the_fly(g.variant)) return false;
+    return true;
+  }
   void cg_step(double *x, const double *dinv, hipStream_t s) {
-    launch_b_direction(segs.ptr, n_segs, st.ptr, r.ptr, dinv, p.ptr, ap.ptr, x, s);
-    apply(p.ptr, ap.ptr, flags.ptr, s);
-    launch_copy_listed<double>(ap.ptr, p.ptr, cons.ptr, n_cons, s);
-    launch_b_pap(segs.ptr, n_segs, st.ptr, p.ptr, ap.ptr, part.ptr, s);
-    launch_b_alpha(st.ptr, inst_seg.ptr, (uint32_t)ops.size(), part.ptr, s);
+    // ap leaves the direction pass with the identity rows set
+    launch_b_direction(segs.ptr, n_segs, st.ptr, r.ptr, dinv, cmask.ptr, p.ptr, ap.ptr, x,
+                       part_c.ptr, s);
+    if (fused_pap()) {
+      for (auto &g : groups) launch_patch_batch(g.patches, p.ptr, ap.ptr, flags.ptr, s, g.ppart.ptr);
+      launch_b_alpha_fused(st.ptr, pap_ranges.ptr, inst_seg.ptr, (uint32_t)ops.size(), part_c.ptr,
+                           s);
+    } else {
+      apply(p.ptr, ap.ptr, flags.ptr, s);
+      launch_b_pap(segs.ptr, n_segs, st.ptr, p.ptr, ap.ptr, part.ptr, s);
+      launch_b_alpha(st.ptr, inst_seg.ptr, (uint32_t)ops.size(), part.ptr, s);
+    }
     launch_b_update(segs.ptr, n_segs, st.ptr, r.ptr, ap.ptr, dinv, part.ptr, part2.ptr, s);
     launch_b_check(st.ptr, inst_seg.ptr, (uint32_t)ops.size(), part.ptr, part2.ptr, flags.ptr,
                    n_active.ptr, s);
@@ -206,7 +225,12 @@ void build_patches(mfx_batch &b, mfx_batch::Group &g, int n, hipStream_t s) {
   g.inst.upload(inst.data(), inst.size());
   g.patches.p = p;
   g.patches.n_patches = (uint32_t)np_total;
-  g.patches.grid = patch_batch_grid(p, (uint32_t)np_total);
+  g.patches.grid = patch_batch_grid(p, (uint32_t)np_total, false);
+  g.patches.grid_dot = patch_batch_grid(p, (uint32_t)np_total, true);
+  g.patches.n_inst = (uint32_t)n;
+  g.waves = patch_batch_waves(p);
+  g.ppart.alloc((uint64_t)n * g.patches.grid_dot * g.waves);
+  launch_fill(g.ppart.ptr, 0.0, g.ppart.n, s);
   g.patches.pmap = g.pmap.ptr;
   g.patches.meta = g.meta.ptr;
   g.patches.inst = g.inst.ptr;
@@ -294,6 +318,21 @@ mfx_status mfx_batch_create(const mfx_batch_instance *inst, int n, int tune, mfx
                         s);
       c0 += b->ops[i]->n_cons;
     }
+    // fused p.Ap: each instance's range of patch partials in its group, and
+    // segments of the constrained list (per instance, at most kSegment rows)
+    {
+      std::vector<PapRange> rg(n);
+      for (int i = 0; i < n; ++i)
+        for (auto &g : b->groups)
+          if (g.p == b->ops[i]->p) {
+            const uint64_t slot = (uint64_t)g.patches.grid_dot * g.waves;
+            rg[i] = PapRange{g.ppart.ptr + (uint64_t)i * slot, slot};
+          }
+      b->pap_ranges.upload(rg.data(), rg.size());
+    }
+    b->cmask.alloc(b->N);
+    MFX_HIP(hipMemsetAsync(b->cmask.ptr, 0, b->N, s));
+    launch_mark_u8(b->cons.ptr, b->n_cons, b->cmask.ptr, s);
     // segments: contiguous DoF ranges of one instance
     std::vector<Seg> segs;
     std::vector<uint32_t> inst_seg(n + 1, 0);
@@ -305,6 +344,7 @@ mfx_status mfx_batch_create(const mfx_batch_instance *inst, int n, int tune, mfx
     inst_seg[n] = (uint32_t)segs.size();
     b->n_segs = (uint32_t)segs.size();
     b->segs.upload(segs.data(), segs.size());
+    b->part_c.alloc(b->n_segs);
     b->inst_seg.upload(inst_seg.data(), inst_seg.size());
     MFX_HIP(hipStreamSynchronize(s));
     if (tune) autotune(*b, s);
@@ -435,6 +475,8 @@ mfx_status mfx_batch_cg_solve(mfx_batch *b, double *x, const double *rhs, const 
       dinv = b->ones.ptr;
     }
     const auto t0 = std::chrono::steady_clock::now();
+    // the fused p.Ap keeps its partial slots zero between steps; start clean
+    for (auto &g : b->groups) launch_fill(g.ppart.ptr, 0.0, g.ppart.n, s);
     b->vmult(b->t.ptr, x, s);
     launch_b_residual(b->segs.ptr, b->n_segs, b->r.ptr, rhs, b->t.ptr, dinv, b->part.ptr,
                       b->part2.ptr, s);

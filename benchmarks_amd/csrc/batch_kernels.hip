@@ -114,23 +114,33 @@ k_b_count(const int *__restrict__ flags, uint32_t n_inst, int *__restrict__ n_ac
 }
 
 // x += alpha p of the previous step (it needs the p read here anyway), then
-// p = dinv r + beta p
+// p = dinv r + beta p; ap = 0 on the free rows (the apply accumulates
+// there) and ap = p on the constrained rows (the identity rows of the
+// operator), whose share of p.ap, the sum of p^2 over them, goes to one
+// partial per segment
 __global__ void __launch_bounds__(kBlock)
 k_b_direction(const Seg *__restrict__ segs, const BatchCg *__restrict__ st,
               const double *__restrict__ r, const double *__restrict__ dinv,
-              double *__restrict__ p, double *__restrict__ ap, double *__restrict__ x) {
+              const uint8_t *__restrict__ cmask, double *__restrict__ p, double *__restrict__ ap,
+              double *__restrict__ x, double *__restrict__ part_c) {
   const Seg sg = segs[blockIdx.x];
   const BatchCg &c = st[sg.inst];
   if (c.done) return;
   const bool first = c.first != 0;
   const double beta = first ? 0.0 : c.beta, alpha = c.alpha;
+  double pc = 0.0, unused = 0.0;
 #pragma unroll 4
   for (uint64_t i = sg.begin + threadIdx.x; i < sg.end; i += kBlock) {
     const double pi = p[i];
     if (!first) x[i] += alpha * pi;
-    p[i] = dinv[i] * r[i] + (beta != 0.0 ? beta * pi : 0.0);
-    ap[i] = 0.0;
+    const double pn = dinv[i] * r[i] + (beta != 0.0 ? beta * pi : 0.0);
+    p[i] = pn;
+    const bool cons = cmask[i] != 0;
+    ap[i] = cons ? pn : 0.0;
+    pc += cons ? pn * pn : 0.0;
   }
+  block_sum2(pc, unused);
+  if (threadIdx.x == 0) part_c[blockIdx.x] = pc;
 }
 
 // the last step's x += alpha p of every instance that took a step
@@ -166,6 +176,31 @@ k_b_alpha(BatchCg *__restrict__ st, const uint32_t *__restrict__ inst_seg,
   instance_sums(inst_seg, i, part, nullptr, pap, unused);
   if (threadIdx.x == 0) {
     BatchCg &c = st[i];
+    if (!(pap > 0.0)) {
+      c.breakdown = 1;
+      c.alpha = 0.0;
+    } else {
+      c.alpha = c.gamma / pap;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_alpha_fused(BatchCg *__restrict__ st, const PapRange *__restrict__ ranges,
+                const uint32_t *__restrict__ inst_seg, const double *__restrict__ part_c) {
+  const uint32_t i = blockIdx.x;
+  if (st[i].done) return;
+  const PapRange rg = ranges[i];
+  double a = 0.0, b = 0.0;
+  for (uint64_t k = threadIdx.x; k < rg.n; k += kBlock) {
+    a += rg.part[k];
+    rg.part[k] = 0.0;  // the patch kernel writes only the slots it uses
+  }
+  for (uint32_t k = inst_seg[i] + threadIdx.x; k < inst_seg[i + 1]; k += kBlock) b += part_c[k];
+  block_sum2(a, b);
+  if (threadIdx.x == 0) {
+    BatchCg &c = st[i];
+    const double pap = a + b;
     if (!(pap > 0.0)) {
       c.breakdown = 1;
       c.alpha = 0.0;
@@ -242,6 +277,12 @@ __global__ void k_fill_u16(uint16_t *__restrict__ dst, uint16_t v, uint64_t n) {
     dst[i] = v;
 }
 
+__global__ void k_mark_u8(const uint32_t *__restrict__ list, uint64_t n, uint8_t *__restrict__ mask) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    mask[list[k]] = 1;
+}
+
 inline unsigned grid_for(uint64_t n) {
   return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + kBlock - 1) / kBlock, 65536));
 }
@@ -267,10 +308,11 @@ void launch_b_init(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const
 }
 
 void launch_b_direction(const Seg *segs, uint32_t n_segs, const BatchCg *st, const double *r,
-                        const double *dinv, double *p, double *ap, double *x, hipStream_t s) {
+                        const double *dinv, const uint8_t *cmask, double *p, double *ap, double *x,
+                        double *part_c, hipStream_t s) {
   if (!n_segs) return;
-  hipLaunchKernelGGL(k_b_direction, dim3(n_segs), dim3(kBlock), 0, s, segs, st, r, dinv, p, ap,
-                     x);
+  hipLaunchKernelGGL(k_b_direction, dim3(n_segs), dim3(kBlock), 0, s, segs, st, r, dinv, cmask, p,
+                     ap, x, part_c);
   MFX_HIP(hipGetLastError());
 }
 
@@ -291,6 +333,19 @@ void launch_b_pap(const Seg *segs, uint32_t n_segs, const BatchCg *st, const dou
 void launch_b_alpha(BatchCg *st, const uint32_t *inst_seg, uint32_t n_inst, const double *part,
                     hipStream_t s) {
   hipLaunchKernelGGL(k_b_alpha, dim3(n_inst), dim3(kBlock), 0, s, st, inst_seg, part);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_b_alpha_fused(BatchCg *st, const PapRange *ranges, const uint32_t *inst_seg,
+                          uint32_t n_inst, const double *part_c, hipStream_t s) {
+  hipLaunchKernelGGL(k_b_alpha_fused, dim3(n_inst), dim3(kBlock), 0, s, st, ranges, inst_seg,
+                     part_c);
+  MFX_HIP(hipGetLastError());
+}
+
+void launch_mark_u8(const uint32_t *list, uint64_t n, uint8_t *mask, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mark_u8, dim3(grid_for(n)), dim3(kBlock), 0, s, list, n, mask);
   MFX_HIP(hipGetLastError());
 }
 

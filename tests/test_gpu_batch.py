@@ -106,15 +106,20 @@ def test_batch_cg_on_the_fly_matches_oracle(gpu):
         assert np.linalg.norm(xh[a:e] - xo) <= 1e-7 * np.linalg.norm(xo), i
 
 
+@pytest.mark.parametrize("form", ["stored", "on_the_fly"])
 @pytest.mark.parametrize("use_graph", [True, False])
 @pytest.mark.parametrize("steps", [0, 1, 3])
-def test_batch_iterate_after_a_step_limit_matches_oracle(gpu, steps, use_graph):
+def test_batch_iterate_after_a_step_limit_matches_oracle(gpu, steps, use_graph, form):
     """Every instance stopped by the step limit holds the oracle's iterate
     after the same number of steps (the deferred x update of the last step is
     applied when the batch stops); with max_steps = 0 no step is taken and x
-    keeps its start value, as deal.II's SolverControl fails at step 0."""
+    keeps its start value, as deal.II's SolverControl fails at step 0.  With
+    every degree on the patch kernel, p.Ap comes from the kernel's per-patch
+    partials plus the identity rows' partials (the fused form)."""
     from benchmarks_amd.laplace import NoConvergence
     B = _batch(SMALL)
+    for p in sorted({p for p, _ in SMALL}):
+        B.set_variant(p, B.tuning(p)["geometry"].index(form))
     b = _rhs(B, 8)
     inv = B.compute_inverse_diagonal()
     x = B.initialize_dof_vector()
