@@ -39,15 +39,18 @@ template <typename T>
 void launch_apply(int p, int nq, bool collocation, const T *B, const T *D,
                   const T *G, const T *src, T *dst, const uint32_t *map,
                   uint32_t n_cells, hipStream_t s, double *dot_partials = nullptr,
-                  const int *skip = nullptr);
+                  const int *skip = nullptr, uint32_t row_cells = 0);
 // Non-collocated (Gauss) apply with the metric terms of the trilinear cells
 // computed per quadrature point from their vertices (cellX [E][8][3]), xq and
-// wq the 1D quadrature points and weights on [0,1].
+// wq the 1D quadrature points and weights on [0,1].  row_cells > 0 (both
+// Gauss applies): the cells form x-rows of row_cells cells numbered x
+// fastest, and the map gives neighbours along x their shared face nodes (a
+// box mesh); the scatter then sums shared x-faces in LDS first.
 template <typename T>
 void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *xq, const T *wq,
                       const double *cellX, const T *src, T *dst, const uint32_t *map,
                       uint32_t n_cells, hipStream_t s, double *dot_partials = nullptr,
-                      const int *skip = nullptr);
+                      const int *skip = nullptr, uint32_t row_cells = 0);
 template <typename T>
 void launch_evec(int p, int nq, bool collocation, const T *B, const T *D,
                  const T *G, const T *in, T *out, uint32_t n_cells,
@@ -64,7 +67,8 @@ void launch_colloc_variant(int p, int v, const double *D, const double *G, const
                            const uint16_t *cell_inst, const int *inst_done, hipStream_t s,
                            int *epb = nullptr, int *prefetch = nullptr);
 // number of partial sums written by launch_apply(..., dot_partials)
-uint32_t apply_n_blocks(int p, int nq, bool collocation, uint32_t n_cells);
+uint32_t apply_n_blocks(int p, int nq, bool collocation, uint32_t n_cells,
+                        uint32_t row_cells = 0);
 
 // set-up kernels (setup.hip)
 struct BoxGeom {

@@ -150,8 +150,12 @@ struct mfx_op {
     o.pbuf = det ? pbuf32.ptr : nullptr;
     return o;
   }
+  // Gauss applies on a box: x-rows of cells (their shared faces are summed
+  // before the scatter); 0 for other meshes and collocation
+  uint32_t row_cells() const { return is_box && !colloc ? box.n[0] : 0; }
   uint32_t apply_blocks() const {
-    return patch ? (det ? grid64.det_dot : grid64.dot) : apply_n_blocks(p, nq, colloc, E);
+    return patch ? (det ? grid64.det_dot : grid64.dot)
+                 : apply_n_blocks(p, nq, colloc, E, row_cells());
   }
   // dst += A_free src (no zeroing, no constrained rows)
   void apply64(const double *src, double *dst, hipStream_t s, double *partials = nullptr,
@@ -160,19 +164,20 @@ struct mfx_op {
       launch_patch<double>(patch_op64(), src, dst, s, partials, skip);
     else if (geometry == MFX_GEOM_ON_THE_FLY)
       launch_apply_otf<double>(p, nq, B.ptr, D.ptr, xq.ptr, wq.ptr, cellX.ptr, src, dst, map.ptr,
-                               E, s, partials, skip);
+                               E, s, partials, skip, row_cells());
     else
       launch_apply<double>(p, nq, colloc, B.ptr, D.ptr, G.ptr, src, dst, map.ptr, E, s, partials,
-                           skip);
+                           skip, row_cells());
   }
   void apply32(const float *src, float *dst, hipStream_t s) const {
     if (patch)
       launch_patch<float>(patch_op32(), src, dst, s);
     else if (geometry == MFX_GEOM_ON_THE_FLY)
       launch_apply_otf<float>(p, nq, B32.ptr, D32.ptr, xq32.ptr, wq32.ptr, cellX.ptr, src, dst,
-                              map.ptr, E, s);
+                              map.ptr, E, s, nullptr, nullptr, row_cells());
     else
-      launch_apply<float>(p, nq, colloc, B32.ptr, D32.ptr, G32.ptr, src, dst, map.ptr, E, s);
+      launch_apply<float>(p, nq, colloc, B32.ptr, D32.ptr, G32.ptr, src, dst, map.ptr, E, s,
+                          nullptr, nullptr, row_cells());
   }
 };
 
