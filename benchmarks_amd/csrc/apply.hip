@@ -373,6 +373,11 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   }
   __syncthreads();
 
+  // p.Ap (DOT): the sum over the cells of u_c^T A_c u_c, formed at the
+  // quadrature points as grad u . (G grad u) while both are in registers
+  // (constrained entries are gathered as zero, so this is the sum over the
+  // free rows of p_i (A p)_i)
+  double dot = 0.0;
   T fz[NQ];
   if (active) {
     T dxr[NQ], dyr[NQ];  // this lane's rows of D
@@ -412,9 +417,12 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
         const T sc = wxy * gwq[z] * recip(det);
 #pragma unroll
         for (int d = 0; d < 3; ++d) W[d] = dx * C0[d] + dy * C1[d] + dz * C2[d];
-        s0[el][z * NQ2 + col] = sc * (C0[0] * W[0] + C0[1] * W[1] + C0[2] * W[2]);
-        s1[el][z * NQ2 + col] = sc * (C1[0] * W[0] + C1[1] * W[1] + C1[2] * W[2]);
+        const T fx = sc * (C0[0] * W[0] + C0[1] * W[1] + C0[2] * W[2]);
+        const T fy = sc * (C1[0] * W[0] + C1[1] * W[1] + C1[2] * W[2]);
+        s0[el][z * NQ2 + col] = fx;
+        s1[el][z * NQ2 + col] = fy;
         fz[z] = sc * (C2[0] * W[0] + C2[1] * W[1] + C2[2] * W[2]);
+        if (DOT) dot += (double)dx * (double)fx + (double)dy * (double)fy + (double)dz * (double)fz[z];
       } else {
         T gxx, gxy, gxz, gyy, gyz, gzz;
         if constexpr (PF) {
@@ -428,9 +436,12 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
           gyz = __builtin_nontemporal_load(Ge + 4 * NQ3 + z * NQ2);
           gzz = __builtin_nontemporal_load(Ge + 5 * NQ3 + z * NQ2);
         }
-        s0[el][z * NQ2 + col] = gxx * dx + gxy * dy + gxz * dz;
-        s1[el][z * NQ2 + col] = gxy * dx + gyy * dy + gyz * dz;
+        const T fx = gxx * dx + gxy * dy + gxz * dz;
+        const T fy = gxy * dx + gyy * dy + gyz * dz;
+        s0[el][z * NQ2 + col] = fx;
+        s1[el][z * NQ2 + col] = fy;
         fz[z] = gxz * dx + gyz * dy + gzz * dz;
+        if (DOT) dot += (double)dx * (double)fx + (double)dy * (double)fy + (double)dz * (double)fz[z];
       }
     }
   }
@@ -486,7 +497,6 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   T *rgt = &s2[0][0];  // ROW: [zn][yn][c], right faces of all but the last cell
   static_assert(!ROW || NM * NM * XW <= EPB * NQ3, "row segment does not fit s1");
   static_assert(!ROW || NM * NM * EPB <= EPB * NQ3, "right faces do not fit s2");
-  double dot = 0.0;
   if (node_col) {
     T bxt[NQ];
 #pragma unroll
@@ -496,7 +506,6 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       T v = 0;
 #pragma unroll
       for (int a = 0; a < NQ; ++a) v += bxt[a] * s0[el][zn * NM * NQ + qy * NQ + a];
-      if (DOT) dot += (double)uin[zn] * (double)v;
       if constexpr (ROW) {
         if (qx < P || (uint32_t)el + 1 == m)
           acc[(zn * NM + qy) * XW + el * P + qx] = v;

@@ -99,7 +99,8 @@ size_t cg_partial_count(const mfx_op &op) {
   const uint32_t nb_apply = op.apply_blocks();
   const uint32_t nb_cons = cg_cons_blocks(op.n_cons);
   const uint32_t nb_vec = cg_vec_blocks(op.N);
-  return std::max<size_t>((size_t)nb_apply + nb_cons, 2 * (size_t)nb_vec);
+  const size_t na = (size_t)nb_apply + nb_cons;
+  return std::max<size_t>(na + cg_alpha_extra((uint32_t)na), 2 * (size_t)nb_vec);
 }
 
 // deal.II SolverCG + ReductionControl on the device (cg_state.hpp): the

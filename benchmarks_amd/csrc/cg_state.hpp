@@ -43,7 +43,10 @@ void cg_direction(const CgState *st, const double *r, const double *invd, double
                   double *x, uint64_t n, hipStream_t s);
 void cg_constrained(const CgState *st, const uint32_t *list, uint64_t nc, const double *p,
                     double *ap, double *part, hipStream_t s);
-void cg_alpha(CgState *st, const double *part, uint32_t n, hipStream_t s);
+// alpha = gamma / (sum of the n partials of p.Ap); with many partials the
+// cg_alpha_extra(n) slots after them hold chunk sums
+void cg_alpha(CgState *st, double *part, uint32_t n, hipStream_t s);
+uint32_t cg_alpha_extra(uint32_t n);
 // r -= alpha Ap with the partials of r.r and r.Pr
 void cg_update(const CgState *st, double *r, const double *ap, const double *invd, uint64_t n,
                double *part, hipStream_t s);
