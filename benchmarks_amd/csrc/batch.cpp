@@ -52,7 +52,9 @@ struct mfx_batch {
     // patch coordinates), instance -> (cells per direction, first cell),
     // concatenated cell vertices
     PatchBatchOp patches;
-    DBuf<double> ppart;  // p.Ap partials of the patch kernel, [instance][workgroup][wave]
+    DBuf<double> ppart;  // p.Ap partials of the patch kernel, [slot][workgroup][wave]
+    DBuf<uint32_t> slot;  // per instance of the batch: its slot (instances of this degree)
+    std::vector<uint32_t> slot_h;
     uint32_t waves = 0;
     DBuf<uint32_t> pmap;
     DBuf<uint4> meta, inst;
@@ -177,6 +179,8 @@ void build_patches(mfx_batch &b, mfx_batch::Group &g, int n, hipStream_t s) {
   const uint64_t P3 = (uint64_t)(sh.bx * p + 1) * (sh.by * p + 1) * (sh.bz * p + 1);
   uint64_t np_total = 0;
   std::vector<uint4> meta, inst(n, uint4{0, 0, 0, 0});
+  g.slot_h.assign(n, 0);
+  uint32_t n_local = 0;
   uint64_t cells = 0;
   for (int i = 0; i < n; ++i) {
     const mfx_op &op = *b.ops[i];
@@ -185,6 +189,7 @@ void build_patches(mfx_batch &b, mfx_batch::Group &g, int n, hipStream_t s) {
     const uint32_t q0 = (n0 + sh.bx - 1) / sh.bx, q1 = (n1 + sh.by - 1) / sh.by,
                    q2 = (n2 + sh.bz - 1) / sh.bz;
     inst[i] = uint4{n0, n1, n2, (uint32_t)cells};
+    g.slot_h[i] = n_local++;
     for (uint32_t z = 0; z < q2; ++z)
       for (uint32_t y = 0; y < q1; ++y)
         for (uint32_t x = 0; x < q0; ++x) meta.push_back(uint4{(uint32_t)i, x, y, z});
@@ -227,9 +232,10 @@ void build_patches(mfx_batch &b, mfx_batch::Group &g, int n, hipStream_t s) {
   g.patches.n_patches = (uint32_t)np_total;
   g.patches.grid = patch_batch_grid(p, (uint32_t)np_total, false);
   g.patches.grid_dot = patch_batch_grid(p, (uint32_t)np_total, true);
-  g.patches.n_inst = (uint32_t)n;
+  g.slot.upload(g.slot_h.data(), g.slot_h.size());
+  g.patches.slot = g.slot.ptr;
   g.waves = patch_batch_waves(p);
-  g.ppart.alloc((uint64_t)n * g.patches.grid_dot * g.waves);
+  g.ppart.alloc((uint64_t)n_local * g.patches.grid_dot * g.waves);
   launch_fill(g.ppart.ptr, 0.0, g.ppart.n, s);
   g.patches.pmap = g.pmap.ptr;
   g.patches.meta = g.meta.ptr;
@@ -325,8 +331,8 @@ mfx_status mfx_batch_create(const mfx_batch_instance *inst, int n, int tune, mfx
       for (int i = 0; i < n; ++i)
         for (auto &g : b->groups)
           if (g.p == b->ops[i]->p) {
-            const uint64_t slot = (uint64_t)g.patches.grid_dot * g.waves;
-            rg[i] = PapRange{g.ppart.ptr + (uint64_t)i * slot, slot};
+            const uint64_t slot_len = (uint64_t)g.patches.grid_dot * g.waves;
+            rg[i] = PapRange{g.ppart.ptr + (uint64_t)g.slot_h[i] * slot_len, slot_len};
           }
       b->pap_ranges.upload(rg.data(), rg.size());
     }

@@ -50,7 +50,7 @@ struct PatchOp {
 struct PatchBatchOp {
   int p = 0;
   uint32_t n_patches = 0;
-  uint32_t n_inst = 0;  // instances of the whole batch
+  const uint32_t *slot = nullptr;  // per instance: its p.Ap slot among this degree's instances
   uint32_t grid = 0, grid_dot = 0;  // without / with the p.Ap partials
   const uint32_t *pmap = nullptr;  // [n_patches][P3]
   const uint4 *meta = nullptr;     // per patch: instance, px, py, pz
@@ -62,11 +62,12 @@ uint32_t patch_batch_grid(int p, uint32_t n_patches, bool dot);
 // waves per workgroup of the batch kernel: p.Ap partials per patch
 uint32_t patch_batch_waves(int p);
 // dst += A src over the batch's patches (FP64, no zeroing).  With `partials`
-// (n_inst x grid_dot x patch_batch_waves(p) doubles, zero on entry) every
-// workgroup also stores, per instance and wave, its sum of src_i (A src)_i
-// over the DoFs of its patches of that instance, and leaves the slots of
-// instances it has no patch of untouched: the n_inst slots of grid_dot x
-// waves partials, each summed, are src.(A src) of each instance's free rows.
+// (one slot of grid_dot x patch_batch_waves(p) doubles per instance of this
+// degree, slot[instance] its index; zero on entry) every workgroup also
+// stores, per instance and wave, its sum of src_i (A src)_i over the DoFs of
+// its patches of that instance, and leaves the slots of instances it has no
+// patch of untouched: each slot, summed, is src.(A src) of that instance's
+// free rows.
 void launch_patch_batch(const PatchBatchOp &o, const double *src, double *dst, const int *done,
                         hipStream_t s, double *partials = nullptr);
 
