@@ -821,6 +821,11 @@ def main():
                  "vmult_frac_of_stored_G_hbm_roofline": {
                      r["p"]: r["frac_of_stored_G_hbm_roofline"] for r in per
                      if "frac_of_stored_G_hbm_roofline" in r},
+                 # the stored-metric apply is memory-bound: its own kernel's
+                 # algorithmic bytes per second against 8 TB/s
+                 "stored_kernel_hbm_frac": {
+                     r["p"]: r["modes"]["stored"]["kernel_GBs"] * 1e9 / HBM_PEAK for r in per
+                     if "stored" in r["modes"]},
                  "ms_per_launch": dm["ms_kernel"], "ms_per_launch_best": dm["ms_kernel_best"],
                  "ms_vmult_median": dm["ms_vmult_median"],
                  "timing": f"median of {dm['kernel_samples']} per-call HIP event pairs on the "

@@ -30,13 +30,22 @@ static void finish_setup(mfx_op *op, int with_fp32, hipStream_t s) {
     const uint64_t P3 = (uint64_t)(sh.bx * op->p + 1) * (sh.by * op->p + 1) * (sh.bz * op->p + 1);
     op->pmap.alloc(npatch * P3);
     launch_patch_map(op->lay, op->p, sh, op->pmap.ptr, s);
-    op->grid64 = patch_grid<double>(op->p, op->lay);
-    if (with_fp32) op->grid32 = patch_grid<float>(op->p, op->lay);
+    const bool stored = op->geometry == MFX_GEOM_STORED;
+    if (stored) {
+      op->Gp.alloc(patch_geometry_size(op->p, op->lay));
+      launch_patch_geometry<double>(op->p, op->lay, op->G.ptr, op->Gp.ptr, s);
+    }
+    op->grid64 = patch_grid<double>(op->p, op->lay, stored);
+    if (with_fp32) op->grid32 = patch_grid<float>(op->p, op->lay, stored);
   }
   if (with_fp32) {
     if (op->geometry == MFX_GEOM_STORED) {
       op->G32.alloc(op->G.n);
       launch_to_float(op->G.ptr, op->G32.ptr, op->G.n, s);
+      if (op->patch) {
+        op->Gp32.alloc(op->Gp.n);
+        launch_patch_geometry<float>(op->p, op->lay, op->G32.ptr, op->Gp32.ptr, s);
+      }
     }
     op->B32.alloc(op->B.n);
     launch_to_float(op->B.ptr, op->B32.ptr, op->B.n, s);
@@ -287,11 +296,9 @@ mfx_status mfx_op_create_box(const mfx_box_desc *d, mfx_op **out) {
       }
     op->n_cons = cons.size();
     op->cons.upload(cons.data(), cons.size());
-    // stored metric terms stream through the per-cell kernels (measured faster
-    // with G); on-the-fly geometry runs the patch kernel for GLL collocation
-    // and the per-cell Gauss kernel with the metric from the cell vertices
-    // otherwise
-    if (op->colloc && d->geometry == MFX_GEOM_ON_THE_FLY) {
+    // GLL collocation runs the patch kernel, with the metric terms from the
+    // cell vertices or from G; Gauss quadrature runs the per-cell kernels
+    if (op->colloc) {
       op->patch = true;
       op->shape = patch_shape(p);
       const int b[3] = {op->shape.bx, op->shape.by, op->shape.bz};

@@ -104,12 +104,14 @@ struct mfx_op {
   DBuf<uint32_t> map, cons;
   DBuf<double> G, B, D, xq, wq, support, cellX;
   DBuf<float> G32, B32, D32, xq32, wq32;
-  // patch-of-cells kernels (box meshes, GLL collocation, on-the-fly geometry)
+  // patch-of-cells kernels (box meshes, GLL collocation)
   bool patch = false;
   PatchLayout lay{};
   PatchShape shape{};
   PatchGrid grid64{}, grid32{};
   DBuf<uint32_t> pmap;
+  DBuf<double> Gp;   // stored geometry: G in the patch kernel's order
+  DBuf<float> Gp32;
   // deterministic scatter (mfx_op_set_deterministic): per-patch sums
   bool det = false;
   DBuf<double> pbuf64;
@@ -132,6 +134,7 @@ struct mfx_op {
     o.w = wq.ptr;
     o.xq = xq.ptr;
     o.cellX = cellX.ptr;
+    o.G = geometry == MFX_GEOM_STORED ? Gp.ptr : nullptr;
     o.pmap = pmap.ptr;
     o.pbuf = det ? pbuf64.ptr : nullptr;
     return o;
@@ -146,6 +149,7 @@ struct mfx_op {
     o.w = wq32.ptr;
     o.xq = xq32.ptr;
     o.cellX = cellX.ptr;
+    o.G = geometry == MFX_GEOM_STORED ? Gp32.ptr : nullptr;
     o.pmap = pmap.ptr;
     o.pbuf = det ? pbuf32.ptr : nullptr;
     return o;

@@ -35,6 +35,7 @@ struct PatchOp {
   const T *w = nullptr;          // 1D quadrature weights [nq]
   const T *xq = nullptr;         // 1D quadrature points [nq]
   const double *cellX = nullptr; // cell vertices [n_cells][8][3], lexicographic cells
+  const T *G = nullptr;          // stored metric terms in patch order (nullptr: from cellX)
   const uint32_t *pmap = nullptr;
   // deterministic scatter: per-patch sums [n_patches][P3], added per node in
   // patch order by a second kernel (nullptr: atomic scatter)
@@ -73,11 +74,17 @@ void launch_patch_batch(const PatchBatchOp &o, const double *src, double *dst, c
 
 PatchShape patch_shape(int p);
 uint32_t patch_count(const PatchLayout &lay);
+// grids of the kernels with the metric from the vertices or (stored) from G
 template <typename T>
-PatchGrid patch_grid(int p, const PatchLayout &lay);
+PatchGrid patch_grid(int p, const PatchLayout &lay, bool stored);
 template <typename T>
 void launch_patch(const PatchOp<T> &o, const T *src, T *dst, hipStream_t s,
                   double *partials = nullptr, const int *skip = nullptr);
 void launch_patch_map(const PatchLayout &lay, int p, PatchShape sh, uint32_t *pmap, hipStream_t s);
+// The stored metric terms G [cell][6][nq^3] permuted into the patch order the
+// kernel with stored geometry reads (patch_geometry_size elements).
+uint64_t patch_geometry_size(int p, const PatchLayout &lay);
+template <typename T>
+void launch_patch_geometry(int p, const PatchLayout &lay, const T *G, T *Gp, hipStream_t s);
 
 }  // namespace mfx

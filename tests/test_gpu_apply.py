@@ -133,27 +133,31 @@ def test_deterministic_scatter_matches_oracle_bit_for_bit_on_repeat(gpu, p, n):
 def test_patch_kernel_over_many_patches_per_workgroup(gpu, p, n):
     """Meshes with several times more patches than the persistent patch
     kernel has workgroups, so that every workgroup walks many patches
-    through its software pipeline (the small meshes of the oracle tests fit
-    one patch per workgroup): the on-the-fly apply agrees with the
-    stored-metric collocation kernel (an independent code path) to 1e-12,
-    and two applies agree to 1e-13 (FP64) and 1e-5 (FP32 against FP64)."""
+    through its software pipeline (the small meshes of the other oracle
+    tests fit one patch per workgroup): the apply with the metric from the
+    vertices and the one with stored metric terms both agree with the oracle
+    to 1e-12, and two applies agree to 1e-13 (FP64) and 1e-5 (FP32 against
+    FP64)."""
     from benchmarks_amd.laplace import LaplaceOperator, random_vector
     print(f"case p={p} n={n} many patches", flush=True)
-    otf = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05, geometry=1, with_fp32=True)
-    assert otf.info.n_patches >= 3 * 256
-    x = random_vector(otf.m(), 4)
-    y1, y2 = otf.initialize_dof_vector(), otf.initialize_dof_vector()
-    otf.vmult(y1, x)
-    otf.vmult(y2, x)
-    assert (torch.linalg.norm(y2 - y1) / torch.linalg.norm(y1)).item() <= 1e-13
-    f = torch.zeros(otf.m(), dtype=torch.float32, device="cuda")
-    otf.vmult(f, x.float())
-    assert (torch.linalg.norm(f.double() - y1) / torch.linalg.norm(y1)).item() <= 1e-5
-    del otf
-    stored = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05, geometry=0)
-    ys = stored.initialize_dof_vector()
-    stored.vmult(ys, x)
-    assert (torch.linalg.norm(ys - y1) / torch.linalg.norm(ys)).item() <= TOL64
+    x = None
+    for geometry in (1, 0):
+        op = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05, geometry=geometry,
+                             with_fp32=True)
+        assert op.info.n_patches >= 3 * 256
+        if x is None:
+            x = random_vector(op.m(), 4)
+            P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
+            ref = P.vmult(x.cpu().numpy())
+        y1, y2 = op.initialize_dof_vector(), op.initialize_dof_vector()
+        op.vmult(y1, x)
+        op.vmult(y2, x)
+        assert (torch.linalg.norm(y2 - y1) / torch.linalg.norm(y1)).item() <= 1e-13
+        assert _rel(y1.cpu().numpy(), ref) <= TOL64
+        f = torch.zeros(op.m(), dtype=torch.float32, device="cuda")
+        op.vmult(f, x.float())
+        assert (torch.linalg.norm(f.double() - y1) / torch.linalg.norm(y1)).item() <= 1e-5
+        del op
 
 
 def test_deterministic_scatter_needs_the_patch_kernel(gpu):
