@@ -390,7 +390,7 @@ mfx_status mfx_op_set_deterministic(mfx_op *op, int on) {
     MFX_REQUIRE(op, "null pointer argument");
     MFX_REQUIRE(!on || op->patch,
                 "deterministic scatter is implemented for the patch kernel (box operators with "
-                "GLL collocation and on-the-fly geometry)");
+                "GLL collocation)");
     MFX_HIP(hipDeviceSynchronize());  // no apply of this operator in flight
     if (on) {
       const PatchShape sh = op->shape;

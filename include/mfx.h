@@ -203,7 +203,7 @@ mfx_status mfx_op_vmult_f32(mfx_op *op, float *dst, const float *src,
  * CG iterations on them -- are bitwise reproducible from run to run.  The
  * reference's scatter is atomic (CEED_bp/include/bk3_kokkos_kernel.h:356-382)
  * and not reproducible.  Operators on the patch kernel only (box meshes,
- * GLL collocation, on-the-fly geometry); holds one value per patch DoF
+ * GLL collocation, either geometry mode); holds one value per patch DoF
  * (FP64, and FP32 with with_fp32) while on.  Synchronizes the device. */
 mfx_status mfx_op_set_deterministic(mfx_op *op, int on);
 

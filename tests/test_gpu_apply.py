@@ -89,16 +89,19 @@ def test_vmult_on_the_fly_geometry_matches_oracle(gpu, p, n):
 
 @pytest.mark.parametrize("p", range(1, 9))
 @pytest.mark.parametrize("n", [(3, 4, 5), (1, 1, 1)])
-def test_deterministic_scatter_matches_oracle_bit_for_bit_on_repeat(gpu, p, n):
+@pytest.mark.parametrize("geometry", [1, 0])
+def test_deterministic_scatter_matches_oracle_bit_for_bit_on_repeat(gpu, p, n, geometry):
     """Deterministic scatter (mfx_op_set_deterministic): repeated applies are
     bitwise identical (FP64 and FP32); they agree with the oracle to 1e-12
     and with the atomic scatter to 1e-13 (only the summation order differs);
     vmult_add adds to dst; switching the mode off restores the atomic form.
     Meshes whose cell counts are not multiples of the patch size leave
-    partial patches, whose nodes the gather must still find."""
+    partial patches, whose nodes the gather must still find.  Both geometry
+    modes (metric from the vertices, stored metric terms)."""
     from benchmarks_amd.laplace import LaplaceOperator
-    print(f"case p={p} n={n} deterministic", flush=True)
-    op = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05, geometry=1, with_fp32=True)
+    print(f"case p={p} n={n} geometry={geometry} deterministic", flush=True)
+    op = LaplaceOperator(p, p + 1, O.GLL, n_cells=n, deform=0.05, geometry=geometry,
+                         with_fp32=True)
     P = O.make_problem(p, p + 1, O.GLL, n, deform=0.05)
     x = torch.from_numpy(O.splitmix64_uniform(1, P.N)).cuda()
     ya = op.initialize_dof_vector()
