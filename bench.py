@@ -88,7 +88,7 @@ def per_call_ms(fn, calls, stream):
     return sorted(a.elapsed_time(b) for a, b in ev)
 
 
-def bench_mode(p, geometry, steps, warmup):
+def bench_mode(p, geometry, steps, warmup, det_timing=True):
     import torch
 
     from benchmarks_amd.laplace import LaplaceOperator, random_vector
@@ -136,7 +136,7 @@ def bench_mode(p, geometry, steps, warmup):
     b_alg = algorithmic_bytes(p, N, E, geometry, tuple(info.patch_cells), info.n_patches)
     f_alg = flops_per_cell(p, geometry) * E
     det = {}
-    if geometry == ON_THE_FLY:
+    if geometry == ON_THE_FLY and det_timing:
         # the price of bitwise reproducibility: the same vmult with the
         # deterministic scatter (patch sums stored, then added per DoF in a
         # fixed order)
@@ -156,7 +156,7 @@ def bench_mode(p, geometry, steps, warmup):
     return op, x, y, res
 
 
-def bench_degree(p, steps, warmup, do_cg, modes):
+def bench_degree(p, steps, warmup, do_cg, modes, det_timing=True):
     import torch
 
     from benchmarks_amd.laplace import (DiagonalMatrix, ReductionControl, SolverCG,
@@ -164,7 +164,7 @@ def bench_degree(p, steps, warmup, do_cg, modes):
     out = {"p": p, "n_cells_1d": CELLS[p], "modes": {}}
     best = None
     for g in modes:
-        op, x, y, r = bench_mode(p, g, steps, warmup)
+        op, x, y, r = bench_mode(p, g, steps, warmup, det_timing)
         out["n_dofs"], out["n_cells"] = op.m(), op.n_cells()
         out["modes"][r["geometry"]] = r
         if best is None or r["ms_vmult"] < best[1]["ms_vmult"]:
@@ -774,6 +774,9 @@ def main():
                          "(default: 2 when --degrees is left at 4,6,8, else none)")
     ap.add_argument("--geometry", choices=["both", "stored", "otf"], default="both")
     ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-deterministic", action="store_true",
+                    help="skip timing the deterministic scatter (one apply kernel per mode, "
+                         "for counter profiles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c1", action="store_true", help="skip the 35937-DoF BP3 unit-cube case")
     ap.add_argument("--no-bp3-reference", action="store_true",
@@ -793,11 +796,13 @@ def main():
     import torch
     torch.cuda.synchronize()
     stream_copy = stream_copy_GBs()
-    per = [bench_degree(p, args.steps, args.warmup, not args.no_cg, modes) for p in degrees]
+    per = [bench_degree(p, args.steps, args.warmup, not args.no_cg, modes, not args.no_deterministic)
+           for p in degrees]
     if args.extra_degrees is None:
         args.extra_degrees = "2" if args.degrees == "4,6,8" else ""
     extra = [int(d) for d in args.extra_degrees.split(",") if d.strip()]
-    per_extra = [bench_degree(p, args.steps, args.warmup, not args.no_cg, modes) for p in extra
+    per_extra = [bench_degree(p, args.steps, args.warmup, not args.no_cg, modes,
+                              not args.no_deterministic) for p in extra
                  if p not in degrees]
     t_sum = sum(r["ms_vmult"] for r in per) * 1e-3
     n_sum = sum(r["n_dofs"] for r in per)
