@@ -262,6 +262,11 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   // interpolation (3-20% faster at p = 7, 8 with q = 9, 10; slower at nq = 7
   // in FP64, no gain below).  Smaller nq loads them in the gradient phase.
   constexpr bool PF = NQ >= 9 && !OTF;
+  // line-form x/y derivatives (below): with the metric from the vertices and
+  // nq = 5..7 (the D entries then come through the scalar cache); measured
+  // 15-19% faster in FP32 at nq = 5, 7 and 1.5-3% in FP64, level at nq = 6
+  // in FP32, slower at nq = 4 (6%) and nq >= 8 (2x, D read from LDS)
+  constexpr bool LINE = OTF && NQ >= 5 && NQ <= 7;
   const T *Ge = OTF ? nullptr : G + (size_t)(active ? e : 0) * 6 * NQ3 + col;
   T g[PF ? 6 : 1][PF ? NQ : 1];
   if constexpr (OTF) {
@@ -377,12 +382,41 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   // quadrature points as grad u . (G grad u) while both are in registers
   // (constrained entries are gathered as zero, so this is the sum over the
   // free rows of p_i (A p)_i)
+  // LINE: the x and y derivatives (and their transposes in 6.) are taken
+  // along whole lines first -- thread (l0, l1) = (qx, qy) of a cell
+  // differentiates the x-line (y = l0, z = l1) and the y-line (x = l0,
+  // z = l1) into s0 / s1, which are free after the interpolation -- so each
+  // point's value is read once per line instead of NQ times per point.
+  if constexpr (LINE) {
+    if (active) {
+      T ln[NQ];
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) ln[a] = s2[el][qy * NQ2 + qx * NQ + a];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        T t = 0;
+#pragma unroll
+        for (int a = 0; a < NQ; ++a) t += uD(i, a) * ln[a];
+        s0[el][qy * NQ2 + qx * NQ + i] = t;
+      }
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) ln[a] = s2[el][qy * NQ2 + a * NQ + qx];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        T t = 0;
+#pragma unroll
+        for (int a = 0; a < NQ; ++a) t += uD(i, a) * ln[a];
+        s1[el][qy * NQ2 + i * NQ + qx] = t;
+      }
+    }
+    __syncthreads();
+  }
   double dot = 0.0;
   T fz[NQ];
   if (active) {
-    T dxr[NQ], dyr[NQ];  // this lane's rows of D
+    T dxr[LINE ? 1 : NQ], dyr[LINE ? 1 : NQ];  // this lane's rows of D
 #pragma unroll
-    for (int a = 0; a < NQ; ++a) {
+    for (int a = 0; a < (LINE ? 0 : NQ); ++a) {
       dxr[a] = sD[qx * NQ + a];
       dyr[a] = sD[qy * NQ + a];
     }
@@ -390,12 +424,18 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
       T dx = 0, dy = 0, dz = 0;
+      if constexpr (LINE) {
+        dx = s0[el][z * NQ2 + col];
+        dy = s1[el][z * NQ2 + col];
+      } else {
 #pragma unroll
-      for (int a = 0; a < NQ; ++a) {
-        dx += dxr[a] * s2[el][z * NQ2 + qy * NQ + a];
-        dy += dyr[a] * s2[el][z * NQ2 + a * NQ + qx];
-        dz += uD(z, a) * uq[a];
+        for (int a = 0; a < NQ; ++a) {
+          dx += dxr[a] * s2[el][z * NQ2 + qy * NQ + a];
+          dy += dyr[a] * s2[el][z * NQ2 + a * NQ + qx];
+        }
       }
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) dz += uD(z, a) * uq[a];
       if constexpr (OTF) {
         const T zeta = gxq[z];
         T J0[3], J1[3], C0[3], C1[3], C2[3], W[3];
@@ -448,22 +488,51 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   __syncthreads();
 
   // 6. transposed gradient, 7. transposed interpolation along z (registers)
-  if (active) {
-    T w[NQ], dxt[NQ], dyt[NQ];  // this lane's columns of D
+  if constexpr (LINE) {
+    if (active) {
+      T ln[NQ];
 #pragma unroll
-    for (int a = 0; a < NQ; ++a) {
+      for (int i = 0; i < NQ; ++i) ln[i] = s0[el][qy * NQ2 + qx * NQ + i];
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        T t = 0;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) t += uD(i, k) * ln[i];
+        s0[el][qy * NQ2 + qx * NQ + k] = t;
+      }
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) ln[i] = s1[el][qy * NQ2 + i * NQ + qx];
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        T t = 0;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) t += uD(i, k) * ln[i];
+        s1[el][qy * NQ2 + k * NQ + qx] = t;
+      }
+    }
+    __syncthreads();
+  }
+  if (active) {
+    T w[NQ], dxt[LINE ? 1 : NQ], dyt[LINE ? 1 : NQ];  // this lane's columns of D
+#pragma unroll
+    for (int a = 0; a < (LINE ? 0 : NQ); ++a) {
       dxt[a] = sD[a * NQ + qx];
       dyt[a] = sD[a * NQ + qy];
     }
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
       T v = 0;
+      if constexpr (LINE) {
+        v = s0[el][z * NQ2 + col] + s1[el][z * NQ2 + col];
+      } else {
 #pragma unroll
-      for (int a = 0; a < NQ; ++a) {
-        v += dxt[a] * s0[el][z * NQ2 + qy * NQ + a];
-        v += dyt[a] * s1[el][z * NQ2 + a * NQ + qx];
-        v += uD(a, z) * fz[a];
+        for (int a = 0; a < NQ; ++a) {
+          v += dxt[a] * s0[el][z * NQ2 + qy * NQ + a];
+          v += dyt[a] * s1[el][z * NQ2 + a * NQ + qx];
+        }
       }
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) v += uD(a, z) * fz[a];
       w[z] = v;
     }
 #pragma unroll
