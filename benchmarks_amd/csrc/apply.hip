@@ -316,8 +316,22 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       Kd[6] = ((x7 - x6) - (x5 - x4)) - a1;
     }
   }
-  // 2. interpolate along x: s1[z][y][qx], z,y < NM
-  if (active && qy < NM) {
+  // 2. interpolate along x: s1[z][y][qx], z,y < NM (LINE: thread (qx, qy)
+  // interpolates the node x-line y = qx, z = qy whole)
+  if constexpr (LINE) {
+    if (active && qx < NM && qy < NM) {
+      T ln[NM];
+#pragma unroll
+      for (int a = 0; a < NM; ++a) ln[a] = s0[el][qy * NM2 + qx * NM + a];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        T t = 0;
+#pragma unroll
+        for (int a = 0; a < NM; ++a) t += uB(i, a) * ln[a];
+        s1[el][qy * NM * NQ + qx * NQ + i] = t;
+      }
+    }
+  } else if (active && qy < NM) {
     T bx[NM];  // this lane's row of B, read from LDS once
 #pragma unroll
     for (int a = 0; a < NM; ++a) bx[a] = sB[qx * NM + a];
@@ -545,8 +559,22 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   }
   __syncthreads();
 
-  // 8. transposed interpolation along y: s0[zn][yn][qx]
-  if (active && qy < NM) {
+  // 8. transposed interpolation along y: s0[zn][yn][qx] (LINE: thread
+  // (qx, qy) takes the y-line x = qx, zn = qy whole)
+  if constexpr (LINE) {
+    if (active && qy < NM) {
+      T ln[NQ];
+#pragma unroll
+      for (int b = 0; b < NQ; ++b) ln[b] = s2[el][qy * NQ2 + b * NQ + qx];
+#pragma unroll
+      for (int yn = 0; yn < NM; ++yn) {
+        T t = 0;
+#pragma unroll
+        for (int b = 0; b < NQ; ++b) t += uB(b, yn) * ln[b];
+        s0[el][qy * NM * NQ + yn * NQ + qx] = t;
+      }
+    }
+  } else if (active && qy < NM) {
     T byt[NQ];
 #pragma unroll
     for (int b = 0; b < NQ; ++b) byt[b] = sB[b * NM + qy];
@@ -566,7 +594,24 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   T *rgt = &s2[0][0];  // ROW: [zn][yn][c], right faces of all but the last cell
   static_assert(!ROW || NM * NM * XW <= EPB * NQ3, "row segment does not fit s1");
   static_assert(!ROW || NM * NM * EPB <= EPB * NQ3, "right faces do not fit s2");
-  if (node_col) {
+  if constexpr (LINE && ROW) {
+    // thread (qx, qy) takes the x-line yn = qx, zn = qy whole
+    if (node_col) {
+      T ln[NQ];
+#pragma unroll
+      for (int a = 0; a < NQ; ++a) ln[a] = s0[el][qy * NM * NQ + qx * NQ + a];
+#pragma unroll
+      for (int xn = 0; xn < NM; ++xn) {
+        T v = 0;
+#pragma unroll
+        for (int a = 0; a < NQ; ++a) v += uB(a, xn) * ln[a];
+        if (xn < P || (uint32_t)el + 1 == m)
+          acc[(qy * NM + qx) * XW + el * P + xn] = v;
+        else
+          rgt[(qy * NM + qx) * EPB + el] = v;
+      }
+    }
+  } else if (node_col) {
     T bxt[NQ];
 #pragma unroll
     for (int a = 0; a < NQ; ++a) bxt[a] = sB[a * NM + qx];
