@@ -262,10 +262,13 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   // interpolation (3-20% faster at p = 7, 8 with q = 9, 10; slower at nq = 7
   // in FP64, no gain below).  Smaller nq loads them in the gradient phase.
   constexpr bool PF = NQ >= 9 && !OTF;
-  // line-form x/y derivatives (below): with the metric from the vertices and
-  // nq = 5..7 (the D entries then come through the scalar cache); measured
-  // 15-19% faster in FP32 at nq = 5, 7 and 1.5-3% in FP64, level at nq = 6
-  // in FP32, slower at nq = 4 (6%) and nq >= 8 (2x, D read from LDS)
+  // LINE: every x and y contraction (interpolation, derivative and their
+  // transposes) is taken along whole lines, one line per thread, instead of
+  // reading a line per output point -- with the metric from the vertices and
+  // nq = 5..7, where the B and D entries come through the scalar cache.
+  // Measured 19-20% faster in FP32 at nq = 5, 7 and 8-11% in FP64 (4% in
+  // FP32 at nq = 6); the derivatives alone were slower at nq = 4 (6%) and
+  // nq >= 8 (2x, D read from LDS); with stored G (memory-bound) level.
   constexpr bool LINE = OTF && NQ >= 5 && NQ <= 7;
   const T *Ge = OTF ? nullptr : G + (size_t)(active ? e : 0) * 6 * NQ3 + col;
   T g[PF ? 6 : 1][PF ? NQ : 1];
@@ -370,15 +373,35 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
 #pragma unroll
       for (int c = 0; c < 6; ++c) g[c][z] = __builtin_nontemporal_load(Ge + c * NQ3 + z * NQ2);
   }
-  if (active) {
-    T t2[NM], by[NM];
+  if constexpr (LINE) {
+    // thread (qx, qy) interpolates the y-line x = qx, z = qy (< NM) into s0
+    if (active && qy < NM) {
+      T ln[NM];
 #pragma unroll
-    for (int b = 0; b < NM; ++b) by[b] = sB[qy * NM + b];
+      for (int c = 0; c < NM; ++c) ln[c] = s1[el][qy * NM * NQ + c * NQ + qx];
+#pragma unroll
+      for (int yq = 0; yq < NQ; ++yq) {
+        T t = 0;
+#pragma unroll
+        for (int c = 0; c < NM; ++c) t += uB(yq, c) * ln[c];
+        s0[el][qy * NQ2 + yq * NQ + qx] = t;
+      }
+    }
+    __syncthreads();
+  }
+  if (active) {
+    T t2[NM], by[LINE ? 1 : NM];
+#pragma unroll
+    for (int b = 0; b < (LINE ? 0 : NM); ++b) by[b] = sB[qy * NM + b];
 #pragma unroll
     for (int z = 0; z < NM; ++z) {
       T t = 0;
+      if constexpr (LINE) {
+        t = s0[el][z * NQ2 + col];
+      } else {
 #pragma unroll
-      for (int b = 0; b < NM; ++b) t += by[b] * s1[el][z * NM * NQ + b * NQ + qx];
+        for (int b = 0; b < NM; ++b) t += by[b] * s1[el][z * NM * NQ + b * NQ + qx];
+      }
       t2[z] = t;
     }
 #pragma unroll
