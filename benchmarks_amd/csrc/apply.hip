@@ -267,9 +267,11 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   // reading a line per output point -- with the metric from the vertices and
   // nq = 5..7, where the B and D entries come through the scalar cache.
   // Measured 19-20% faster in FP32 at nq = 5, 7 and 8-11% in FP64 (4% in
-  // FP32 at nq = 6); the derivatives alone were slower at nq = 4 (6%) and
-  // nq >= 8 (2x, D read from LDS); with stored G (memory-bound) level.
-  constexpr bool LINE = OTF && NQ >= 5 && NQ <= 7;
+  // FP32 at nq = 6), 1-2% at nq = 3 and 5% in FP32 at nq = 4, where FP64 was
+  // 2-3% slower; the derivatives alone were 2x slower at nq >= 8 (D read
+  // from LDS); with stored G (memory-bound) level.
+  constexpr bool LINE =
+      OTF && ((NQ >= 5 && NQ <= 7) || NQ == 3 || (NQ == 4 && std::is_same<T, float>::value));
   const T *Ge = OTF ? nullptr : G + (size_t)(active ? e : 0) * 6 * NQ3 + col;
   T g[PF ? 6 : 1][PF ? NQ : 1];
   if constexpr (OTF) {
