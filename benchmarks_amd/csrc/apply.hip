@@ -213,7 +213,8 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
                T *__restrict__ dst, const uint32_t *__restrict__ map,
                uint32_t n_cells, double *__restrict__ partials,
                const int *__restrict__ skip, const double *__restrict__ cellX,
-               const T *__restrict__ gxq, const T *__restrict__ gwq, uint32_t row_cells) {
+               const T *__restrict__ gxq, const T *__restrict__ gwq, uint32_t row_cells,
+               const T *__restrict__ geo) {
   constexpr int EPB = Tile<NQ>::EPB, NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
   constexpr int NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int THREADS = Tile<NQ>::THREADS;
@@ -272,6 +273,10 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   // from LDS); with stored G (memory-bound) level.
   constexpr bool LINE =
       OTF && ((NQ >= 5 && NQ <= 7) || NQ == 3 || (NQ == 4 && std::is_same<T, float>::value));
+  // the line-form contractions apply B, B^T, D, D^T through their even-odd
+  // tables (half the multiply-adds; geo holds the four in that order)
+  constexpr int EO_BT = eo_table_size(NQ, NM), EO_D = EO_BT + eo_table_size(NM, NQ),
+                EO_DT = EO_D + eo_table_size(NQ, NQ);
   const T *Ge = OTF ? nullptr : G + (size_t)(active ? e : 0) * 6 * NQ3 + col;
   T g[PF ? 6 : 1][PF ? NQ : 1];
   if constexpr (OTF) {
@@ -328,13 +333,10 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       T ln[NM];
 #pragma unroll
       for (int a = 0; a < NM; ++a) ln[a] = s0[el][qy * NM2 + qx * NM + a];
+      T o[NQ];
+      eo_tab<NQ, NM, 1>(geo, ln, o);
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) {
-        T t = 0;
-#pragma unroll
-        for (int a = 0; a < NM; ++a) t += uB(i, a) * ln[a];
-        s1[el][qy * NM * NQ + qx * NQ + i] = t;
-      }
+      for (int i = 0; i < NQ; ++i) s1[el][qy * NM * NQ + qx * NQ + i] = o[i];
     }
   } else if (active && qy < NM) {
     T bx[NM];  // this lane's row of B, read from LDS once
@@ -381,13 +383,10 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       T ln[NM];
 #pragma unroll
       for (int c = 0; c < NM; ++c) ln[c] = s1[el][qy * NM * NQ + c * NQ + qx];
+      T o[NQ];
+      eo_tab<NQ, NM, 1>(geo, ln, o);
 #pragma unroll
-      for (int yq = 0; yq < NQ; ++yq) {
-        T t = 0;
-#pragma unroll
-        for (int c = 0; c < NM; ++c) t += uB(yq, c) * ln[c];
-        s0[el][qy * NQ2 + yq * NQ + qx] = t;
-      }
+      for (int yq = 0; yq < NQ; ++yq) s0[el][qy * NQ2 + yq * NQ + qx] = o[yq];
     }
     __syncthreads();
   }
@@ -406,13 +405,19 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       }
       t2[z] = t;
     }
+    if constexpr (LINE) {
+      eo_tab<NQ, NM, 1>(geo, t2, uq);
 #pragma unroll
-    for (int zq = 0; zq < NQ; ++zq) {
-      T t = 0;
+      for (int zq = 0; zq < NQ; ++zq) s2[el][zq * NQ2 + col] = uq[zq];
+    } else {
 #pragma unroll
-      for (int c = 0; c < NM; ++c) t += uB(zq, c) * t2[c];
-      uq[zq] = t;
-      s2[el][zq * NQ2 + col] = t;
+      for (int zq = 0; zq < NQ; ++zq) {
+        T t = 0;
+#pragma unroll
+        for (int c = 0; c < NM; ++c) t += uB(zq, c) * t2[c];
+        uq[zq] = t;
+        s2[el][zq * NQ2 + col] = t;
+      }
     }
   }
   __syncthreads();
@@ -431,22 +436,15 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       T ln[NQ];
 #pragma unroll
       for (int a = 0; a < NQ; ++a) ln[a] = s2[el][qy * NQ2 + qx * NQ + a];
+      T o[NQ];
+      eo_tab<NQ, NQ, -1>(geo + EO_D, ln, o);
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) {
-        T t = 0;
-#pragma unroll
-        for (int a = 0; a < NQ; ++a) t += uD(i, a) * ln[a];
-        s0[el][qy * NQ2 + qx * NQ + i] = t;
-      }
+      for (int i = 0; i < NQ; ++i) s0[el][qy * NQ2 + qx * NQ + i] = o[i];
 #pragma unroll
       for (int a = 0; a < NQ; ++a) ln[a] = s2[el][qy * NQ2 + a * NQ + qx];
+      eo_tab<NQ, NQ, -1>(geo + EO_D, ln, o);
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) {
-        T t = 0;
-#pragma unroll
-        for (int a = 0; a < NQ; ++a) t += uD(i, a) * ln[a];
-        s1[el][qy * NQ2 + i * NQ + qx] = t;
-      }
+      for (int i = 0; i < NQ; ++i) s1[el][qy * NQ2 + i * NQ + qx] = o[i];
     }
     __syncthreads();
   }
@@ -460,21 +458,23 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       dyr[a] = sD[qy * NQ + a];
     }
     const T wxy = OTF ? gwq[qx] * gwq[qy] : T(0);
+    T dzl[LINE ? NQ : 1];  // LINE: the column's z derivatives
+    if constexpr (LINE) eo_tab<NQ, NQ, -1>(geo + EO_D, uq, dzl);
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
       T dx = 0, dy = 0, dz = 0;
       if constexpr (LINE) {
         dx = s0[el][z * NQ2 + col];
         dy = s1[el][z * NQ2 + col];
+        dz = dzl[z];
       } else {
 #pragma unroll
         for (int a = 0; a < NQ; ++a) {
           dx += dxr[a] * s2[el][z * NQ2 + qy * NQ + a];
           dy += dyr[a] * s2[el][z * NQ2 + a * NQ + qx];
+          dz += uD(z, a) * uq[a];
         }
       }
-#pragma unroll
-      for (int a = 0; a < NQ; ++a) dz += uD(z, a) * uq[a];
       if constexpr (OTF) {
         const T zeta = gxq[z];
         T J0[3], J1[3], C0[3], C1[3], C2[3], W[3];
@@ -532,22 +532,15 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       T ln[NQ];
 #pragma unroll
       for (int i = 0; i < NQ; ++i) ln[i] = s0[el][qy * NQ2 + qx * NQ + i];
+      T o[NQ];
+      eo_tab<NQ, NQ, -1>(geo + EO_DT, ln, o);
 #pragma unroll
-      for (int k = 0; k < NQ; ++k) {
-        T t = 0;
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) t += uD(i, k) * ln[i];
-        s0[el][qy * NQ2 + qx * NQ + k] = t;
-      }
+      for (int k = 0; k < NQ; ++k) s0[el][qy * NQ2 + qx * NQ + k] = o[k];
 #pragma unroll
       for (int i = 0; i < NQ; ++i) ln[i] = s1[el][qy * NQ2 + i * NQ + qx];
+      eo_tab<NQ, NQ, -1>(geo + EO_DT, ln, o);
 #pragma unroll
-      for (int k = 0; k < NQ; ++k) {
-        T t = 0;
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) t += uD(i, k) * ln[i];
-        s1[el][qy * NQ2 + k * NQ + qx] = t;
-      }
+      for (int k = 0; k < NQ; ++k) s1[el][qy * NQ2 + k * NQ + qx] = o[k];
     }
     __syncthreads();
   }
@@ -558,28 +551,36 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       dxt[a] = sD[a * NQ + qx];
       dyt[a] = sD[a * NQ + qy];
     }
+    T dzt[LINE ? NQ : 1];  // LINE: D^T of the column's z fluxes
+    if constexpr (LINE) eo_tab<NQ, NQ, -1>(geo + EO_DT, fz, dzt);
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
       T v = 0;
       if constexpr (LINE) {
-        v = s0[el][z * NQ2 + col] + s1[el][z * NQ2 + col];
+        v = s0[el][z * NQ2 + col] + s1[el][z * NQ2 + col] + dzt[z];
       } else {
 #pragma unroll
         for (int a = 0; a < NQ; ++a) {
           v += dxt[a] * s0[el][z * NQ2 + qy * NQ + a];
           v += dyt[a] * s1[el][z * NQ2 + a * NQ + qx];
+          v += uD(a, z) * fz[a];
         }
       }
-#pragma unroll
-      for (int a = 0; a < NQ; ++a) v += uD(a, z) * fz[a];
       w[z] = v;
     }
+    if constexpr (LINE) {
+      T o[NM];
+      eo_tab<NM, NQ, 1>(geo + EO_BT, w, o);
 #pragma unroll
-    for (int zn = 0; zn < NM; ++zn) {
-      T t = 0;
+      for (int zn = 0; zn < NM; ++zn) s2[el][zn * NQ2 + col] = o[zn];
+    } else {
 #pragma unroll
-      for (int c = 0; c < NQ; ++c) t += uB(c, zn) * w[c];
-      s2[el][zn * NQ2 + col] = t;
+      for (int zn = 0; zn < NM; ++zn) {
+        T t = 0;
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) t += uB(c, zn) * w[c];
+        s2[el][zn * NQ2 + col] = t;
+      }
     }
   }
   __syncthreads();
@@ -591,13 +592,10 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       T ln[NQ];
 #pragma unroll
       for (int b = 0; b < NQ; ++b) ln[b] = s2[el][qy * NQ2 + b * NQ + qx];
+      T o[NM];
+      eo_tab<NM, NQ, 1>(geo + EO_BT, ln, o);
 #pragma unroll
-      for (int yn = 0; yn < NM; ++yn) {
-        T t = 0;
-#pragma unroll
-        for (int b = 0; b < NQ; ++b) t += uB(b, yn) * ln[b];
-        s0[el][qy * NM * NQ + yn * NQ + qx] = t;
-      }
+      for (int yn = 0; yn < NM; ++yn) s0[el][qy * NM * NQ + yn * NQ + qx] = o[yn];
     }
   } else if (active && qy < NM) {
     T byt[NQ];
@@ -625,11 +623,11 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
       T ln[NQ];
 #pragma unroll
       for (int a = 0; a < NQ; ++a) ln[a] = s0[el][qy * NM * NQ + qx * NQ + a];
+      T o[NM];
+      eo_tab<NM, NQ, 1>(geo + EO_BT, ln, o);
 #pragma unroll
       for (int xn = 0; xn < NM; ++xn) {
-        T v = 0;
-#pragma unroll
-        for (int a = 0; a < NQ; ++a) v += uB(a, xn) * ln[a];
+        const T v = o[xn];
         if (xn < P || (uint32_t)el + 1 == m)
           acc[(qy * NM + qx) * XW + el * P + xn] = v;
         else
@@ -720,25 +718,25 @@ void launch_t(const T *B, const T *D, const T *G, const T *src, T *dst,
         if (row)
           hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, true, false, true>), grid, block,
                              0, s, B, D, G, src, dst, map, n_cells, partials, skip, nullptr,
-                             nullptr, nullptr, row_cells);
+                             nullptr, nullptr, row_cells, (const T *)nullptr);
         else
           hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, true>), grid, block, 0, s, B, D,
                              G, src, dst, map, n_cells, partials, skip, nullptr, nullptr, nullptr,
-                             0u);
+                             0u, (const T *)nullptr);
       }
     } else {
       if constexpr (GATHER) {
         if (row) {
           hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, false, false, true>), grid,
                              block, 0, s, B, D, G, src, dst, map, n_cells, nullptr, nullptr,
-                             nullptr, nullptr, nullptr, row_cells);
+                             nullptr, nullptr, nullptr, row_cells, (const T *)nullptr);
           MFX_HIP(hipGetLastError());
           return;
         }
       }
       hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, GATHER, false>), grid, block, 0, s,
                          B, D, G, src, dst, map, n_cells, nullptr, nullptr, nullptr, nullptr,
-                         nullptr, 0u);
+                         nullptr, 0u, (const T *)nullptr);
     }
   }
   MFX_HIP(hipGetLastError());
@@ -855,9 +853,10 @@ void launch_evec(int p, int nq, bool colloc, const T *B, const T *D, const T *G,
 // Gauss (non-collocated) apply with the metric terms computed from the cell
 // vertices (k_apply_interp<..., OTF>).
 template <typename T, int NM, int NQ>
-void launch_interp_otf(const T *B, const T *D, const T *xq, const T *wq, const double *cellX,
-                       const T *src, T *dst, const uint32_t *map, uint32_t n_cells,
-                       hipStream_t s, double *partials, const int *skip, uint32_t row_cells) {
+void launch_interp_otf(const T *B, const T *D, const T *eo, const T *xq, const T *wq,
+                       const double *cellX, const T *src, T *dst, const uint32_t *map,
+                       uint32_t n_cells, hipStream_t s, double *partials, const int *skip,
+                       uint32_t row_cells) {
   if (n_cells == 0) return;
   MFX_REQUIRE(!row_cells || n_cells % row_cells == 0, "row segments need whole x-rows of cells");
   const dim3 grid(n_blocks_for<NQ, false>(n_cells, row_cells)), block(Tile<NQ>::THREADS);
@@ -866,38 +865,40 @@ void launch_interp_otf(const T *B, const T *D, const T *xq, const T *wq, const d
       if (row_cells)
         hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, true, true, true>), grid, block, 0, s,
                            B, D, nullptr, src, dst, map, n_cells, partials, skip, cellX, xq, wq,
-                           row_cells);
+                           row_cells, eo);
       else
         hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, true, true>), grid, block, 0, s, B, D,
-                           nullptr, src, dst, map, n_cells, partials, skip, cellX, xq, wq, 0u);
+                           nullptr, src, dst, map, n_cells, partials, skip, cellX, xq, wq, 0u,
+                           eo);
     } else {
       throw Error{MFX_ERR_UNSUPPORTED, "p.Ap partials are computed in FP64 only"};
     }
   } else if (row_cells) {
     hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, false, true, true>), grid, block, 0, s, B,
                        D, nullptr, src, dst, map, n_cells, nullptr, nullptr, cellX, xq, wq,
-                       row_cells);
+                       row_cells, eo);
   } else {
     hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, false, true>), grid, block, 0, s, B, D,
-                       nullptr, src, dst, map, n_cells, nullptr, nullptr, cellX, xq, wq, 0u);
+                       nullptr, src, dst, map, n_cells, nullptr, nullptr, cellX, xq, wq, 0u,
+                       eo);
   }
   MFX_HIP(hipGetLastError());
 }
 
 template <typename T>
-void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *xq, const T *wq,
-                      const double *cellX, const T *src, T *dst, const uint32_t *map,
-                      uint32_t n_cells, hipStream_t s, double *partials, const int *skip,
-                      uint32_t row_cells) {
-  MFX_REQUIRE(B && D && xq && wq && cellX, "on-the-fly apply: missing tables or vertices");
+void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *eo, const T *xq,
+                      const T *wq, const double *cellX, const T *src, T *dst,
+                      const uint32_t *map, uint32_t n_cells, hipStream_t s, double *partials,
+                      const int *skip, uint32_t row_cells) {
+  MFX_REQUIRE(B && D && eo && xq && wq && cellX, "on-the-fly apply: missing tables or vertices");
 #define MFX_CASE_O(P)                                                                         \
   case P:                                                                                    \
     if (nq == P + 1)                                                                         \
-      return launch_interp_otf<T, P + 1, P + 1>(B, D, xq, wq, cellX, src, dst, map, n_cells, \
-                                                s, partials, skip, row_cells);               \
+      return launch_interp_otf<T, P + 1, P + 1>(B, D, eo, xq, wq, cellX, src, dst, map,      \
+                                                n_cells, s, partials, skip, row_cells);      \
     if (nq == P + 2)                                                                         \
-      return launch_interp_otf<T, P + 1, P + 2>(B, D, xq, wq, cellX, src, dst, map, n_cells, \
-                                                s, partials, skip, row_cells);               \
+      return launch_interp_otf<T, P + 1, P + 2>(B, D, eo, xq, wq, cellX, src, dst, map,      \
+                                                n_cells, s, partials, skip, row_cells);      \
     break;
   switch (p) {
     MFX_CASE_O(1) MFX_CASE_O(2) MFX_CASE_O(3) MFX_CASE_O(4)
@@ -910,12 +911,12 @@ void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *xq, const 
 }
 
 template void launch_apply_otf<double>(int, int, const double *, const double *, const double *,
-                                       const double *, const double *, const double *, double *,
-                                       const uint32_t *, uint32_t, hipStream_t, double *,
-                                       const int *, uint32_t);
+                                       const double *, const double *, const double *,
+                                       const double *, double *, const uint32_t *, uint32_t,
+                                       hipStream_t, double *, const int *, uint32_t);
 template void launch_apply_otf<float>(int, int, const float *, const float *, const float *,
-                                      const float *, const double *, const float *, float *,
-                                      const uint32_t *, uint32_t, hipStream_t, double *,
+                                      const float *, const float *, const double *, const float *,
+                                      float *, const uint32_t *, uint32_t, hipStream_t, double *,
                                       const int *, uint32_t);
 
 template void launch_apply<double>(int, int, bool, const double *, const double *,

@@ -51,6 +51,10 @@ static void finish_setup(mfx_op *op, int with_fp32, hipStream_t s) {
     launch_to_float(op->B.ptr, op->B32.ptr, op->B.n, s);
     op->D32.alloc(op->D.n);
     launch_to_float(op->D.ptr, op->D32.ptr, op->D.n, s);
+    if (op->eo.n) {
+      op->eo32.alloc(op->eo.n);
+      launch_to_float(op->eo.ptr, op->eo32.ptr, op->eo.n, s);
+    }
     op->xq32.alloc(op->xq.n);
     launch_to_float(op->xq.ptr, op->xq32.ptr, op->xq.n, s);
     op->wq32.alloc(op->wq.n);
@@ -81,6 +85,22 @@ static void common_init(mfx_op *op, int p, int nq, int quad, int geometry) {
   }
   op->B.upload(B.data(), B.size());
   op->D.upload(D.data(), D.size());
+  if (!op->colloc) {
+    // even-odd tables of the line-form Gauss kernel: B and D on symmetric
+    // points are centro-symmetric and centro-antisymmetric
+    const int nm = p + 1;
+    std::vector<double> Bt((size_t)nm * nq), Dt((size_t)nq * nq), eo;
+    for (int q = 0; q < nq; ++q)
+      for (int m = 0; m < nm; ++m) Bt[(size_t)m * nq + q] = B[(size_t)q * nm + m];
+    for (int i = 0; i < nq; ++i)
+      for (int j = 0; j < nq; ++j) Dt[(size_t)j * nq + i] = D[(size_t)i * nq + j];
+    double dev = eo_tables(B.data(), nq, nm, 1, eo);
+    dev = std::max(dev, eo_tables(Bt.data(), nm, nq, 1, eo));
+    dev = std::max(dev, eo_tables(D.data(), nq, nq, -1, eo));
+    dev = std::max(dev, eo_tables(Dt.data(), nq, nq, -1, eo));
+    MFX_REQUIRE(dev <= 1e-13, "basis tables are not centro-symmetric");
+    op->eo.upload(eo.data(), eo.size());
+  }
   op->xq.upload(xq.data(), xq.size());
   op->wq.upload(wq.data(), wq.size());
   op->support.upload(sup.data(), sup.size());

@@ -46,11 +46,14 @@ void launch_apply(int p, int nq, bool collocation, const T *B, const T *D,
 // Gauss applies): the cells form x-rows of row_cells cells numbered x
 // fastest, and the map gives neighbours along x their shared face nodes (a
 // box mesh); the scatter then sums shared x-faces in LDS first.
+// eo: the even-odd tables of B, B^T, D, D^T (tables.hpp eo_tables) for the
+// line-form contractions.
 template <typename T>
-void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *xq, const T *wq,
-                      const double *cellX, const T *src, T *dst, const uint32_t *map,
-                      uint32_t n_cells, hipStream_t s, double *dot_partials = nullptr,
-                      const int *skip = nullptr, uint32_t row_cells = 0);
+void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *eo, const T *xq,
+                      const T *wq, const double *cellX, const T *src, T *dst,
+                      const uint32_t *map, uint32_t n_cells, hipStream_t s,
+                      double *dot_partials = nullptr, const int *skip = nullptr,
+                      uint32_t row_cells = 0);
 template <typename T>
 void launch_evec(int p, int nq, bool collocation, const T *B, const T *D,
                  const T *G, const T *in, T *out, uint32_t n_cells,

@@ -104,6 +104,10 @@ struct mfx_op {
   DBuf<uint32_t> map, cons;
   DBuf<double> G, B, D, xq, wq, support, cellX;
   DBuf<float> G32, B32, D32, xq32, wq32;
+  // Gauss quadrature: the even-odd tables of B, B^T, D and D^T (eo_tab
+  // layout, in that order) for the line-form contractions
+  DBuf<double> eo;
+  DBuf<float> eo32;
   // patch-of-cells kernels (box meshes, GLL collocation)
   bool patch = false;
   PatchLayout lay{};
@@ -167,8 +171,8 @@ struct mfx_op {
     if (patch)
       launch_patch<double>(patch_op64(), src, dst, s, partials, skip);
     else if (geometry == MFX_GEOM_ON_THE_FLY)
-      launch_apply_otf<double>(p, nq, B.ptr, D.ptr, xq.ptr, wq.ptr, cellX.ptr, src, dst, map.ptr,
-                               E, s, partials, skip, row_cells());
+      launch_apply_otf<double>(p, nq, B.ptr, D.ptr, eo.ptr, xq.ptr, wq.ptr, cellX.ptr, src, dst,
+                               map.ptr, E, s, partials, skip, row_cells());
     else
       launch_apply<double>(p, nq, colloc, B.ptr, D.ptr, G.ptr, src, dst, map.ptr, E, s, partials,
                            skip, row_cells());
@@ -177,8 +181,8 @@ struct mfx_op {
     if (patch)
       launch_patch<float>(patch_op32(), src, dst, s);
     else if (geometry == MFX_GEOM_ON_THE_FLY)
-      launch_apply_otf<float>(p, nq, B32.ptr, D32.ptr, xq32.ptr, wq32.ptr, cellX.ptr, src, dst,
-                              map.ptr, E, s, nullptr, nullptr, row_cells());
+      launch_apply_otf<float>(p, nq, B32.ptr, D32.ptr, eo32.ptr, xq32.ptr, wq32.ptr, cellX.ptr,
+                              src, dst, map.ptr, E, s, nullptr, nullptr, row_cells());
     else
       launch_apply<float>(p, nq, colloc, B32.ptr, D32.ptr, G32.ptr, src, dst, map.ptr, E, s,
                           nullptr, nullptr, row_cells());

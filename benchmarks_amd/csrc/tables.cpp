@@ -36,4 +36,24 @@ double gll_d_table_error(int n, const double *D) {
   }
 }
 
+double eo_tables(const double *M, int O, int I, int S, std::vector<double> &out) {
+  const int HO = O / 2, HI = I / 2, mi = I - 1;
+  double mx = 0.0, dev = 0.0;
+  for (int i = 0; i < O; ++i)
+    for (int a = 0; a < I; ++a) {
+      mx = std::max(mx, std::fabs(M[i * I + a]));
+      dev = std::max(dev, std::fabs(M[i * I + a] - S * M[(O - 1 - i) * I + (I - 1 - a)]));
+    }
+  for (int i = 0; i < HO; ++i)
+    for (int a = 0; a < HI; ++a) out.push_back(0.5 * (M[i * I + a] + S * M[i * I + mi - a]));
+  for (int i = 0; i < HO; ++i)
+    for (int a = 0; a < HI; ++a) out.push_back(0.5 * (M[i * I + a] - S * M[i * I + mi - a]));
+  if (I & 1)
+    for (int i = 0; i < HO; ++i) out.push_back(M[i * I + HI]);
+  if (O & 1)
+    for (int a = 0; a < HI; ++a) out.push_back(M[HO * I + a]);
+  if ((I & 1) && (O & 1)) out.push_back(M[HO * I + HI]);
+  return mx > 0.0 ? dev / mx : 0.0;
+}
+
 }  // namespace mfx

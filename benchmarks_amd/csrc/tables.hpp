@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace mfx {
 
@@ -146,8 +147,67 @@ __device__ __forceinline__ void eo_mat(M mat, const T *in, T *out) {
   }
 }
 
+// Even-odd tables of a centro-symmetric (S = 1) or centro-antisymmetric
+// (S = -1) O x I matrix M, as eo_mat forms them, stored in this order:
+//   cf[i][a] = (M[i][a] + S M[i][I-1-a]) / 2, i < O/2, a < I/2
+//   cg[i][a] = (M[i][a] - S M[i][I-1-a]) / 2
+//   M[i][I/2] (I odd), i < O/2;  M[O/2][a] (O odd), a < I/2;
+//   M[O/2][I/2] (both odd).
+constexpr int eo_table_size(int O, int I) {
+  return 2 * (O / 2) * (I / 2) + ((I & 1) ? O / 2 : 0) + ((O & 1) ? I / 2 : 0) +
+         (((I & 1) && (O & 1)) ? 1 : 0);
+}
+
+// out = M in with M given by its even-odd tables (eo_table_size(O, I)
+// values at tab, read with compile-time offsets, so through the scalar cache)
+template <int O, int I, int S, typename T>
+__device__ __forceinline__ void eo_tab(const T *__restrict__ tab, const T *in, T *out) {
+  static_assert(S == 1 || S == -1, "centro-symmetry sign");
+  constexpr int HI = I / 2, HO = O / 2, mi = I - 1, mo = O - 1;
+  constexpr int CG = HO * HI, MC = 2 * HO * HI, MR = MC + ((I & 1) ? HO : 0), CT = MR + HI;
+  T e[HI > 0 ? HI : 1], o[HI > 0 ? HI : 1];
+#pragma unroll
+  for (int a = 0; a < HI; ++a) {
+    e[a] = in[a] + in[mi - a];
+    o[a] = in[a] - in[mi - a];
+  }
+#pragma unroll
+  for (int i = 0; i < HO; ++i) {
+    T p = 0, q = 0;
+#pragma unroll
+    for (int a = 0; a < HI; ++a) {
+      if constexpr (S == 1) {
+        p += tab[i * HI + a] * e[a];
+        q += tab[CG + i * HI + a] * o[a];
+      } else {
+        p += tab[i * HI + a] * o[a];
+        q += tab[CG + i * HI + a] * e[a];
+      }
+    }
+    if constexpr (I & 1) {
+      if constexpr (S == 1)
+        p += tab[MC + i] * in[HI];
+      else
+        q += tab[MC + i] * in[HI];
+    }
+    out[i] = p + q;
+    out[mo - i] = p - q;
+  }
+  if constexpr (O & 1) {
+    T v = 0;
+#pragma unroll
+    for (int a = 0; a < HI; ++a) v += tab[MR + a] * (S == 1 ? e[a] : o[a]);
+    if constexpr ((I & 1) && S == 1) v += tab[CT] * in[HI];
+    out[HO] = v;
+  }
+}
+
 // max |table - run-time| of the compile-time GLL derivative matrix of n
 // points (D row-major n x n)
 double gll_d_table_error(int n, const double *D);
+// Even-odd tables (eo_tab's layout) of the row-major O x I matrix M with
+// centro-symmetry sign S, appended to out; returns the largest deviation of
+// M from that symmetry relative to its largest entry.
+double eo_tables(const double *M, int O, int I, int S, std::vector<double> &out);
 
 }  // namespace mfx
