@@ -265,14 +265,14 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
   constexpr bool PF = NQ >= 9 && !OTF;
   // LINE: every x and y contraction (interpolation, derivative and their
   // transposes) is taken along whole lines, one line per thread, instead of
-  // reading a line per output point -- with the metric from the vertices and
-  // nq = 5..7, where the B and D entries come through the scalar cache.
-  // Measured 19-20% faster in FP32 at nq = 5, 7 and 8-11% in FP64 (4% in
-  // FP32 at nq = 6), 1-2% at nq = 3 and 5% in FP32 at nq = 4, where FP64 was
-  // 2-3% slower; the derivatives alone were 2x slower at nq >= 8 (D read
-  // from LDS); with stored G (memory-bound) level.
+  // reading a line per output point, with the metric from the vertices; the
+  // line contractions read the even-odd tables of B and D (below) through
+  // the scalar cache.  Measured against the per-point form: 25-45% faster in
+  // FP32 and 10-40% in FP64 at nq = 5..10 (the gain grows with nq), 1-5% at
+  // nq = 3 and in FP32 at nq = 4; FP64 at nq = 4 was 3% slower.  With stored
+  // G (memory-bound) level, so not used there.
   constexpr bool LINE =
-      OTF && ((NQ >= 5 && NQ <= 7) || NQ == 3 || (NQ == 4 && std::is_same<T, float>::value));
+      OTF && (NQ >= 5 || NQ == 3 || (NQ == 4 && std::is_same<T, float>::value));
   // the line-form contractions apply B, B^T, D, D^T through their even-odd
   // tables (half the multiply-adds; geo holds the four in that order)
   constexpr int EO_BT = eo_table_size(NQ, NM), EO_D = EO_BT + eo_table_size(NM, NQ),
