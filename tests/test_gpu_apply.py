@@ -203,6 +203,32 @@ def test_gauss_on_the_fly_geometry_matches_oracle(gpu, p, nq):
     assert _rel(y2.cpu().numpy(), ref) <= TOL64
 
 
+@pytest.mark.parametrize("p,nq", [(p, p + 2) for p in range(1, 9)] +
+                         [(p, p + 1) for p in (2, 3, 4, 6, 8)])
+def test_gauss_on_the_fly_over_full_row_segments(gpu, p, nq):
+    """x-rows of 17 cells: every row holds full segments of the cells a
+    workgroup takes (7 at nq = 6, 2 at nq = 10) and a partial last one, so
+    the row-segment sums of shared x-faces and the line-form contractions run
+    on whole segments, which the small ragged mesh above does not reach; FP64
+    to 1e-12 and FP32 to 1e-5 of the oracle, identity rows bit-exact."""
+    from benchmarks_amd.laplace import LaplaceOperator
+    n = (17, 3, 2)
+    print(f"case p={p} nq={nq} Gauss on-the-fly, rows of 17 cells", flush=True)
+    op = LaplaceOperator(p, nq, O.GAUSS, n_cells=n, deform=0.05, geometry=1, with_fp32=True)
+    P = O.make_problem(p, nq, O.GAUSS, n, deform=0.05)
+    x = O.splitmix64_uniform(3, P.N)
+    xt = torch.from_numpy(x).cuda()
+    y = op.initialize_dof_vector()
+    op.vmult(y, xt)
+    yg, ref = y.cpu().numpy(), P.vmult(x)
+    assert _rel(yg, ref) <= TOL64, _rel(yg, ref)
+    cons = P.constrained.astype(bool)
+    assert np.array_equal(yg[cons], x[cons])
+    y32 = torch.zeros(op.m(), dtype=torch.float32, device="cuda")
+    op.vmult(y32, xt.float())
+    assert _rel(y32.double().cpu().numpy(), ref) <= 1e-5
+
+
 @pytest.mark.parametrize("p", [2, 4, 6, 8])
 def test_patch_map_covers_every_dof(gpu, p):
     """Patch DoF map: every unconstrained DoF appears, constrained ones never;
