@@ -460,6 +460,24 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
     const T wxy = OTF ? gwq[qx] * gwq[qy] : T(0);
     T dzl[LINE ? NQ : 1];  // LINE: the column's z derivatives
     if constexpr (LINE) eo_tab<NQ, NQ, -1>(geo + EO_D, uq, dzl);
+    // the column's points; AFF: an affine cell (da = db = 0 exactly, as on
+    // an undeformed box) has the same cofactor columns and det J at every
+    // point of the column, so they are formed once, by the same operations
+    auto column = [&](auto aff_tag) {
+    constexpr bool AFF = decltype(aff_tag)::value;
+    T H0[3], H1[3], H2[3], hr = T(0);
+    if constexpr (AFF) {
+      H0[0] = cb0[1] * cjz[2] - cb0[2] * cjz[1];
+      H0[1] = cb0[2] * cjz[0] - cb0[0] * cjz[2];
+      H0[2] = cb0[0] * cjz[1] - cb0[1] * cjz[0];
+      H1[0] = cjz[1] * ca0[2] - cjz[2] * ca0[1];
+      H1[1] = cjz[2] * ca0[0] - cjz[0] * ca0[2];
+      H1[2] = cjz[0] * ca0[1] - cjz[1] * ca0[0];
+      H2[0] = ca0[1] * cb0[2] - ca0[2] * cb0[1];
+      H2[1] = ca0[2] * cb0[0] - ca0[0] * cb0[2];
+      H2[2] = ca0[0] * cb0[1] - ca0[1] * cb0[0];
+      hr = recip(cjz[0] * H2[0] + cjz[1] * H2[1] + cjz[2] * H2[2]);
+    }
 #pragma unroll
     for (int z = 0; z < NQ; ++z) {
       T dx = 0, dy = 0, dz = 0;
@@ -476,24 +494,35 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
         }
       }
       if constexpr (OTF) {
-        const T zeta = gxq[z];
-        T J0[3], J1[3], C0[3], C1[3], C2[3], W[3];
+        T C0[3], C1[3], C2[3], W[3], sc;
+        if constexpr (AFF) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          J0[d] = ca0[d] + zeta * cda[d];
-          J1[d] = cb0[d] + zeta * cdb[d];
+          for (int d = 0; d < 3; ++d) {
+            C0[d] = H0[d];
+            C1[d] = H1[d];
+            C2[d] = H2[d];
+          }
+          sc = wxy * gwq[z] * hr;
+        } else {
+          const T zeta = gxq[z];
+          T J0[3], J1[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            J0[d] = ca0[d] + zeta * cda[d];
+            J1[d] = cb0[d] + zeta * cdb[d];
+          }
+          C0[0] = J1[1] * cjz[2] - J1[2] * cjz[1];
+          C0[1] = J1[2] * cjz[0] - J1[0] * cjz[2];
+          C0[2] = J1[0] * cjz[1] - J1[1] * cjz[0];
+          C1[0] = cjz[1] * J0[2] - cjz[2] * J0[1];
+          C1[1] = cjz[2] * J0[0] - cjz[0] * J0[2];
+          C1[2] = cjz[0] * J0[1] - cjz[1] * J0[0];
+          C2[0] = J0[1] * J1[2] - J0[2] * J1[1];
+          C2[1] = J0[2] * J1[0] - J0[0] * J1[2];
+          C2[2] = J0[0] * J1[1] - J0[1] * J1[0];
+          const T det = cjz[0] * C2[0] + cjz[1] * C2[1] + cjz[2] * C2[2];
+          sc = wxy * gwq[z] * recip(det);
         }
-        C0[0] = J1[1] * cjz[2] - J1[2] * cjz[1];
-        C0[1] = J1[2] * cjz[0] - J1[0] * cjz[2];
-        C0[2] = J1[0] * cjz[1] - J1[1] * cjz[0];
-        C1[0] = cjz[1] * J0[2] - cjz[2] * J0[1];
-        C1[1] = cjz[2] * J0[0] - cjz[0] * J0[2];
-        C1[2] = cjz[0] * J0[1] - cjz[1] * J0[0];
-        C2[0] = J0[1] * J1[2] - J0[2] * J1[1];
-        C2[1] = J0[2] * J1[0] - J0[0] * J1[2];
-        C2[2] = J0[0] * J1[1] - J0[1] * J1[0];
-        const T det = cjz[0] * C2[0] + cjz[1] * C2[1] + cjz[2] * C2[2];
-        const T sc = wxy * gwq[z] * recip(det);
 #pragma unroll
         for (int d = 0; d < 3; ++d) W[d] = dx * C0[d] + dy * C1[d] + dz * C2[d];
         const T fx = sc * (C0[0] * W[0] + C0[1] * W[1] + C0[2] * W[2]);
@@ -522,6 +551,18 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
         fz[z] = gxz * dx + gyz * dy + gzz * dz;
         if (DOT) dot += (double)dx * (double)fx + (double)dy * (double)fy + (double)dz * (double)fz[z];
       }
+    }
+    };
+    if constexpr (OTF) {
+      bool aff = true;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) aff = aff && cda[d] == T(0) && cdb[d] == T(0);
+      if (aff)
+        column(std::true_type{});
+      else
+        column(std::false_type{});
+    } else {
+      column(std::false_type{});
     }
   }
   __syncthreads();
