@@ -206,7 +206,8 @@ k_apply_colloc(const T *__restrict__ gD, const T *__restrict__ G,
 // FP atomics execute beyond the per-XCD L2s, and their request count bounds
 // the per-cell scatter (measured: plain stores in their place made the FP32
 // q = p+1 apply at 135M DoFs 2.1x faster).
-template <typename T, int NM, int NQ, bool GATHER, bool DOT, bool OTF = false, bool ROW = false>
+template <typename T, int NM, int NQ, bool GATHER, bool DOT, bool OTF = false, bool ROW = false,
+          bool AX = false>
 __global__ void __launch_bounds__(Tile<NQ>::THREADS)
 k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
                const T *__restrict__ G, const T *__restrict__ src,
@@ -460,13 +461,12 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
     const T wxy = OTF ? gwq[qx] * gwq[qy] : T(0);
     T dzl[LINE ? NQ : 1];  // LINE: the column's z derivatives
     if constexpr (LINE) eo_tab<NQ, NQ, -1>(geo + EO_D, uq, dzl);
-    // the column's points; AFF: an affine cell (da = db = 0 exactly, as on
-    // an undeformed box) has the same cofactor columns and det J at every
-    // point of the column, so they are formed once, by the same operations
-    auto column = [&](auto aff_tag) {
-    constexpr bool AFF = decltype(aff_tag)::value;
-    T H0[3], H1[3], H2[3], hr = T(0);
-    if constexpr (AFF) {
+    // the column's points.  AX (cells of an undeformed box, whose Jacobian
+    // columns lie along x, y and z): the cofactor columns and det J are the
+    // same at every point of the column and diagonal, so they are formed once
+    // and the products with their zero entries are left out
+    [[maybe_unused]] T H0[3], H1[3], H2[3], hr = T(0);
+    if constexpr (OTF && AX) {
       H0[0] = cb0[1] * cjz[2] - cb0[2] * cjz[1];
       H0[1] = cb0[2] * cjz[0] - cb0[0] * cjz[2];
       H0[2] = cb0[0] * cjz[1] - cb0[1] * cjz[0];
@@ -494,18 +494,15 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
         }
       }
       if constexpr (OTF) {
-        T C0[3], C1[3], C2[3], W[3], sc;
-        if constexpr (AFF) {
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            C0[d] = H0[d];
-            C1[d] = H1[d];
-            C2[d] = H2[d];
-          }
-          sc = wxy * gwq[z] * hr;
+        T fx, fy, fzv;
+        if constexpr (AX) {
+          const T sc = wxy * gwq[z] * hr;
+          fx = sc * (H0[0] * (dx * H0[0]));
+          fy = sc * (H1[1] * (dy * H1[1]));
+          fzv = sc * (H2[2] * (dz * H2[2]));
         } else {
           const T zeta = gxq[z];
-          T J0[3], J1[3];
+          T J0[3], J1[3], C0[3], C1[3], C2[3], W[3];
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             J0[d] = ca0[d] + zeta * cda[d];
@@ -521,15 +518,16 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
           C2[1] = J0[2] * J1[0] - J0[0] * J1[2];
           C2[2] = J0[0] * J1[1] - J0[1] * J1[0];
           const T det = cjz[0] * C2[0] + cjz[1] * C2[1] + cjz[2] * C2[2];
-          sc = wxy * gwq[z] * recip(det);
-        }
+          const T sc = wxy * gwq[z] * recip(det);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) W[d] = dx * C0[d] + dy * C1[d] + dz * C2[d];
-        const T fx = sc * (C0[0] * W[0] + C0[1] * W[1] + C0[2] * W[2]);
-        const T fy = sc * (C1[0] * W[0] + C1[1] * W[1] + C1[2] * W[2]);
+          for (int d = 0; d < 3; ++d) W[d] = dx * C0[d] + dy * C1[d] + dz * C2[d];
+          fx = sc * (C0[0] * W[0] + C0[1] * W[1] + C0[2] * W[2]);
+          fy = sc * (C1[0] * W[0] + C1[1] * W[1] + C1[2] * W[2]);
+          fzv = sc * (C2[0] * W[0] + C2[1] * W[1] + C2[2] * W[2]);
+        }
         s0[el][z * NQ2 + col] = fx;
         s1[el][z * NQ2 + col] = fy;
-        fz[z] = sc * (C2[0] * W[0] + C2[1] * W[1] + C2[2] * W[2]);
+        fz[z] = fzv;
         if (DOT) dot += (double)dx * (double)fx + (double)dy * (double)fy + (double)dz * (double)fz[z];
       } else {
         T gxx, gxy, gxz, gyy, gyz, gzz;
@@ -551,18 +549,6 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
         fz[z] = gxz * dx + gyz * dy + gzz * dz;
         if (DOT) dot += (double)dx * (double)fx + (double)dy * (double)fy + (double)dz * (double)fz[z];
       }
-    }
-    };
-    if constexpr (OTF) {
-      bool aff = true;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) aff = aff && cda[d] == T(0) && cdb[d] == T(0);
-      if (aff)
-        column(std::true_type{});
-      else
-        column(std::false_type{});
-    } else {
-      column(std::false_type{});
     }
   }
   __syncthreads();
@@ -897,13 +883,18 @@ template <typename T, int NM, int NQ>
 void launch_interp_otf(const T *B, const T *D, const T *eo, const T *xq, const T *wq,
                        const double *cellX, const T *src, T *dst, const uint32_t *map,
                        uint32_t n_cells, hipStream_t s, double *partials, const int *skip,
-                       uint32_t row_cells) {
+                       uint32_t row_cells, bool axes) {
+  MFX_REQUIRE(!axes || row_cells, "axis-aligned cells are a box-mesh form");
   if (n_cells == 0) return;
   MFX_REQUIRE(!row_cells || n_cells % row_cells == 0, "row segments need whole x-rows of cells");
   const dim3 grid(n_blocks_for<NQ, false>(n_cells, row_cells)), block(Tile<NQ>::THREADS);
   if (partials) {
     if constexpr (std::is_same<T, double>::value) {
-      if (row_cells)
+      if (axes)
+        hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, true, true, true, true>), grid, block,
+                           0, s, B, D, nullptr, src, dst, map, n_cells, partials, skip, cellX, xq,
+                           wq, row_cells, eo);
+      else if (row_cells)
         hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, true, true, true>), grid, block, 0, s,
                            B, D, nullptr, src, dst, map, n_cells, partials, skip, cellX, xq, wq,
                            row_cells, eo);
@@ -914,6 +905,10 @@ void launch_interp_otf(const T *B, const T *D, const T *eo, const T *xq, const T
     } else {
       throw Error{MFX_ERR_UNSUPPORTED, "p.Ap partials are computed in FP64 only"};
     }
+  } else if (axes) {
+    hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, false, true, true, true>), grid, block, 0,
+                       s, B, D, nullptr, src, dst, map, n_cells, nullptr, nullptr, cellX, xq, wq,
+                       row_cells, eo);
   } else if (row_cells) {
     hipLaunchKernelGGL((k_apply_interp<T, NM, NQ, true, false, true, true>), grid, block, 0, s, B,
                        D, nullptr, src, dst, map, n_cells, nullptr, nullptr, cellX, xq, wq,
@@ -930,16 +925,16 @@ template <typename T>
 void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *eo, const T *xq,
                       const T *wq, const double *cellX, const T *src, T *dst,
                       const uint32_t *map, uint32_t n_cells, hipStream_t s, double *partials,
-                      const int *skip, uint32_t row_cells) {
+                      const int *skip, uint32_t row_cells, bool axes) {
   MFX_REQUIRE(B && D && eo && xq && wq && cellX, "on-the-fly apply: missing tables or vertices");
 #define MFX_CASE_O(P)                                                                         \
   case P:                                                                                    \
     if (nq == P + 1)                                                                         \
       return launch_interp_otf<T, P + 1, P + 1>(B, D, eo, xq, wq, cellX, src, dst, map,      \
-                                                n_cells, s, partials, skip, row_cells);      \
+                                                n_cells, s, partials, skip, row_cells, axes); \
     if (nq == P + 2)                                                                         \
       return launch_interp_otf<T, P + 1, P + 2>(B, D, eo, xq, wq, cellX, src, dst, map,      \
-                                                n_cells, s, partials, skip, row_cells);      \
+                                                n_cells, s, partials, skip, row_cells, axes); \
     break;
   switch (p) {
     MFX_CASE_O(1) MFX_CASE_O(2) MFX_CASE_O(3) MFX_CASE_O(4)
@@ -954,11 +949,11 @@ void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *eo, const 
 template void launch_apply_otf<double>(int, int, const double *, const double *, const double *,
                                        const double *, const double *, const double *,
                                        const double *, double *, const uint32_t *, uint32_t,
-                                       hipStream_t, double *, const int *, uint32_t);
+                                       hipStream_t, double *, const int *, uint32_t, bool);
 template void launch_apply_otf<float>(int, int, const float *, const float *, const float *,
                                       const float *, const float *, const double *, const float *,
                                       float *, const uint32_t *, uint32_t, hipStream_t, double *,
-                                      const int *, uint32_t);
+                                      const int *, uint32_t, bool);
 
 template void launch_apply<double>(int, int, bool, const double *, const double *,
                                    const double *, const double *, double *,

@@ -48,12 +48,14 @@ void launch_apply(int p, int nq, bool collocation, const T *B, const T *D,
 // box mesh); the scatter then sums shared x-faces in LDS first.
 // eo: the even-odd tables of B, B^T, D, D^T (tables.hpp eo_tables) for the
 // line-form contractions.
+// axes: the cells are those of an undeformed box (Jacobian columns along x,
+// y and z; row_cells > 0), whose diagonal metric is formed once per column.
 template <typename T>
 void launch_apply_otf(int p, int nq, const T *B, const T *D, const T *eo, const T *xq,
                       const T *wq, const double *cellX, const T *src, T *dst,
                       const uint32_t *map, uint32_t n_cells, hipStream_t s,
                       double *dot_partials = nullptr, const int *skip = nullptr,
-                      uint32_t row_cells = 0);
+                      uint32_t row_cells = 0, bool axes = false);
 template <typename T>
 void launch_evec(int p, int nq, bool collocation, const T *B, const T *D,
                  const T *G, const T *in, T *out, uint32_t n_cells,

@@ -161,6 +161,8 @@ struct mfx_op {
   // Gauss applies on a box: x-rows of cells (their shared faces are summed
   // before the scatter); 0 for other meshes and collocation
   uint32_t row_cells() const { return is_box && !colloc ? box.n[0] : 0; }
+  // an undeformed box: every cell's Jacobian columns lie along x, y and z
+  bool axis_cells() const { return is_box && !colloc && box.deform == 0.0; }
   uint32_t apply_blocks() const {
     return patch ? (det ? grid64.det_dot : grid64.dot)
                  : apply_n_blocks(p, nq, colloc, E, row_cells());
@@ -172,7 +174,7 @@ struct mfx_op {
       launch_patch<double>(patch_op64(), src, dst, s, partials, skip);
     else if (geometry == MFX_GEOM_ON_THE_FLY)
       launch_apply_otf<double>(p, nq, B.ptr, D.ptr, eo.ptr, xq.ptr, wq.ptr, cellX.ptr, src, dst,
-                               map.ptr, E, s, partials, skip, row_cells());
+                               map.ptr, E, s, partials, skip, row_cells(), axis_cells());
     else
       launch_apply<double>(p, nq, colloc, B.ptr, D.ptr, G.ptr, src, dst, map.ptr, E, s, partials,
                            skip, row_cells());
@@ -182,7 +184,8 @@ struct mfx_op {
       launch_patch<float>(patch_op32(), src, dst, s);
     else if (geometry == MFX_GEOM_ON_THE_FLY)
       launch_apply_otf<float>(p, nq, B32.ptr, D32.ptr, eo32.ptr, xq32.ptr, wq32.ptr, cellX.ptr,
-                              src, dst, map.ptr, E, s, nullptr, nullptr, row_cells());
+                              src, dst, map.ptr, E, s, nullptr, nullptr, row_cells(),
+                              axis_cells());
     else
       launch_apply<float>(p, nq, colloc, B32.ptr, D32.ptr, G32.ptr, src, dst, map.ptr, E, s,
                           nullptr, nullptr, row_cells());

@@ -230,17 +230,18 @@ def test_gauss_on_the_fly_over_full_row_segments(gpu, p, nq):
 
 
 @pytest.mark.parametrize("p,nq", [(1, 3), (2, 4), (4, 5), (4, 6), (6, 8), (8, 10)])
-def test_gauss_on_the_fly_affine_and_mixed_cells(gpu, p, nq):
-    """The on-the-fly Gauss kernel forms the cofactors and det J once per
-    column on cells whose Jacobian is constant (an undeformed box) and per
-    point elsewhere.  An undeformed box (all cells affine, row segments) and
-    a mesh from host arrays in which every third cell has a moved vertex
-    (affine and trilinear cells in one workgroup, so both forms run in one
-    wavefront) each match the oracle: FP64 to 1e-12, FP32 to 1e-5."""
+def test_gauss_on_the_fly_undeformed_sheared_and_mixed_cells(gpu, p, nq):
+    """On an undeformed box the on-the-fly Gauss kernel forms the diagonal
+    cofactors and det J once per column (the cells' Jacobian columns lie
+    along x, y and z); every other mesh forms the metric per point.  The box
+    (row segments), a sheared box from host arrays (affine cells, none
+    axis-aligned; dyadic coordinates and shear, so the cells are affine to
+    the last bit) and a mesh from host arrays in which every third box cell
+    has a moved vertex each match the oracle: FP64 to 1e-12, FP32 to 1e-5."""
     import dataclasses
     from benchmarks_amd.laplace import LaplaceOperator
     n = (6, 4, 3)
-    print(f"case p={p} nq={nq} Gauss on-the-fly, affine and mixed cells", flush=True)
+    print(f"case p={p} nq={nq} Gauss on-the-fly, undeformed, sheared and mixed cells", flush=True)
     P = O.make_problem(p, nq, O.GAUSS, n, lo=(-1.0, -0.5, 0.0), hi=(2.8, 1.5, 0.9))
     x = O.splitmix64_uniform(4, P.N)
     xt = torch.from_numpy(x).cuda()
@@ -263,6 +264,31 @@ def test_gauss_on_the_fly_affine_and_mixed_cells(gpu, p, nq):
         op.vmult(y32, xt.float())
         assert _rel(y32.double().cpu().numpy(), ref) <= 1e-5
     assert _rel(Pm.vmult(x), P.vmult(x)) > 1e-4  # the moved vertices change the operator
+    # sheared box: X -> S X + t on the dyadic unit-cube grid of 4 x 4 x 2 cells
+    Q = O.make_problem(p, nq, O.GAUSS, (4, 4, 2))
+    S = np.array([[1.0, 0.25, 0.125], [0.125, 1.0, 0.25], [0.25, -0.125, 1.0]])
+    V = Q.cellX.reshape(Q.E, 8, 3) @ S.T + np.array([0.5, -0.25, 1.0])
+    a1 = V[:, 3] - V[:, 2] - V[:, 1] + V[:, 0]
+    k1 = V[:, 5] - V[:, 1] - V[:, 4] + V[:, 0]
+    k2 = V[:, 6] - V[:, 2] - V[:, 4] + V[:, 0]
+    assert not a1.any() and not k1.any() and not k2.any()  # affine to the last bit
+    assert (V[:, 1] - V[:, 0])[:, 1:].all()  # and not axis-aligned
+    cellXs = V.reshape(-1)
+    Gs = np.zeros_like(Q.G)
+    O.lib().orc_geometry(Q.E, cellXs, nq, Q.xq, Q.wq, Gs)
+    Qs = dataclasses.replace(Q, cellX=cellXs, G=Gs)
+    sheared = LaplaceOperator.from_arrays(p, nq, O.GAUSS, Q.map, cellXs,
+                                          np.nonzero(Q.constrained)[0].astype(np.uint32), Q.N,
+                                          with_fp32=True, geometry=1)
+    xs = O.splitmix64_uniform(5, Q.N)
+    ref = Qs.vmult(xs)
+    assert _rel(ref, Q.vmult(xs)) > 1e-3  # the shear changes the operator
+    y = sheared.initialize_dof_vector()
+    sheared.vmult(y, torch.from_numpy(xs).cuda())
+    assert _rel(y.cpu().numpy(), ref) <= TOL64, _rel(y.cpu().numpy(), ref)
+    y32 = torch.zeros(sheared.m(), dtype=torch.float32, device="cuda")
+    sheared.vmult(y32, torch.from_numpy(xs).cuda().float())
+    assert _rel(y32.double().cpu().numpy(), ref) <= 1e-5
 
 
 @pytest.mark.parametrize("p", [2, 4, 6, 8])
