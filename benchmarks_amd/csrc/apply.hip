@@ -217,6 +217,7 @@ k_apply_interp(const T *__restrict__ gB, const T *__restrict__ gD,
                const T *__restrict__ gxq, const T *__restrict__ gwq, uint32_t row_cells,
                const T *__restrict__ geo) {
   constexpr int EPB = Tile<NQ>::EPB, NQ2 = NQ * NQ, NQ3 = NQ2 * NQ;
+  static_assert(!AX || (OTF && ROW), "axis-aligned cells: a box-mesh form of the vertex metric");
   constexpr int NM2 = NM * NM, NM3 = NM2 * NM;
   constexpr int THREADS = Tile<NQ>::THREADS;
   __shared__ T sB[NQ * NM];
